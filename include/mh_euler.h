@@ -1,0 +1,173 @@
+/* mh_euler.h -- C ABI of the MI355X (gfx950) MUSCL-Hancock Euler solver.
+ *
+ * Plain C: opaque handles, plain pointers and sizes, int status codes; no
+ * exceptions and no exit() cross this boundary.  Every entry point takes an
+ * explicit HIP stream (pass 0 for the null stream) and, unless it says it
+ * synchronises, only enqueues work, so sequences of calls are
+ * graph-capturable.
+ *
+ * The reference (HarryKKKKK/large_sys_eq_opt) has no C ABI; each entry point
+ * replaces the C++ free function or method cited beside it (paths relative
+ * to the reference repository).
+ *
+ * Host state arrays use the reference layout: (ny+4)*(nx+4) cells with two
+ * ghost layers, flat index j*(nx+4)+i (head/cpu/grid_cpu.hpp:86-88); "AoS"
+ * means Conserved{rho,rhou,rhov,E} per cell (head/types.hpp:13-23), "SoA"
+ * means four planes rho, rhou, rhov, E (head/gpu/grid_gpu.cuh:43-46).
+ * On the device a grid is four FP64 planes with a padded row pitch.
+ */
+#ifndef MH_EULER_H
+#define MH_EULER_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct mh_grid_s* mh_grid;
+typedef struct mh_stepper_s* mh_stepper;
+typedef struct mh_batch_s* mh_batch;
+typedef struct mh_adjoint_s* mh_adjoint;
+typedef void* mh_stream; /* hipStream_t */
+
+/* Status codes (the reference prints and calls std::exit instead:
+ * src/gpu/solver_gpu.cu:41-49). */
+enum {
+    MH_OK = 0,
+    MH_ERR_INVALID = 1,   /* bad argument, mismatched grids, unknown solver */
+    MH_ERR_OOM = 2,       /* device allocation failed */
+    MH_ERR_LAUNCH = 3,    /* kernel launch or HIP runtime failure */
+    MH_ERR_WAVESPEED = 4, /* max signal speed <= 0: no finite dt (the CPU
+                             reference throws, src/cpu/solver_cpu.cpp:304-306) */
+    MH_ERR_NODEVICE = 5   /* no HIP device visible */
+};
+
+/* Riemann solvers, numbered as head/riemann.hpp:13-18. */
+enum { MH_HLL = 0, MH_HLLC = 1, MH_EXACT = 2, MH_FORCE = 3 };
+
+/* Boundary kinds per edge.  TRANSMISSIVE = zero-gradient copy of the nearest
+ * interior cell into both ghost layers (head/cpu/boundary_cpu.hpp:5-36);
+ * INFLOW = fixed (Dirichlet) conserved state in the ghost layers. */
+enum { MH_BC_TRANSMISSIVE = 0, MH_BC_INFLOW = 1 };
+enum { MH_EDGE_LEFT = 0, MH_EDGE_RIGHT = 1, MH_EDGE_BOTTOM = 2, MH_EDGE_TOP = 3 };
+
+typedef struct {
+    int kind[4];          /* per edge, MH_EDGE_* order */
+    double state[4][4];   /* conserved (rho, rhou, rhov, E) for INFLOW edges */
+} mh_bc;
+
+typedef struct {
+    int nx, ny;                        /* interior cells */
+    double x_min, x_max, y_min, y_max; /* domain; dx = (x_max-x_min)/nx */
+} mh_grid_desc;
+
+/* Last error message of the calling thread ("" when none). */
+const char* mh_last_error(void);
+/* Library version string. */
+const char* mh_version(void);
+/* Selects the device for the calling thread (hipSetDevice). */
+int mh_set_device(int device);
+
+/* ---- grids: replaces Grid2DGPU (head/gpu/grid_gpu.cuh:11-154) ---------- */
+
+/* Grid2DGPU::Grid2DGPU / allocate (head/gpu/grid_gpu.cuh:15-47); ng is 2. */
+int mh_grid_create(const mh_grid_desc* desc, mh_grid* out);
+/* Grid2DGPU::~Grid2DGPU / release (head/gpu/grid_gpu.cuh:31, :49-59). */
+int mh_grid_destroy(mh_grid g);
+/* Grid2DGPU::nx() ... dy() accessors (head/gpu/grid_gpu.cuh:61-75); pitch is
+ * the device row pitch in doubles, plane_stride the distance between planes. */
+int mh_grid_info(mh_grid g, mh_grid_desc* desc, double* dx, double* dy, int64_t* pitch, int64_t* plane_stride);
+/* Device address of cell (i, j) of plane c, i, j in [-2, n+2) (interior
+ * origin at 0).  For zero-copy interop (e.g. torch tensors over the planes). */
+int mh_grid_plane_ptr(mh_grid g, int c, double** ptr);
+/* Grid2DGPU::upload_from_aos / download_to_aos (head/gpu/grid_gpu.cuh:90-121):
+ * host AoS in the reference layout, ghosts included.  Synchronise. */
+int mh_grid_upload_aos(mh_grid g, const double* host_aos);
+int mh_grid_download_aos(mh_grid g, double* host_aos);
+/* The same with four host SoA planes of (ny+4)*(nx+4) doubles each. */
+int mh_grid_upload_soa(mh_grid g, const double* const planes[4]);
+int mh_grid_download_soa(mh_grid g, double* const planes[4]);
+/* Device-to-device copy of the interior (nx*ny per plane) from/to a dense
+ * device buffer laid out [4][ny][nx].  Enqueued on `stream`. */
+int mh_grid_copy_interior_from_device(mh_grid g, const double* src, mh_stream stream);
+int mh_grid_copy_interior_to_device(mh_grid g, double* dst, mh_stream stream);
+/* Grid2DGPU::swap (head/gpu/grid_gpu.cuh:123-128): exchanges storage. */
+int mh_grid_swap(mh_grid a, mh_grid b);
+
+/* ---- boundary: replaces apply_transmissive_boundary_gpu
+ *      (head/gpu/boundary_gpu.cuh:5, src/gpu/boundary_gpu.cu:71-84) -------- */
+int mh_fill_ghosts(mh_grid g, const mh_bc* bc, mh_stream stream);
+
+/* ---- time step: replaces compute_dt_gpu (head/gpu/solver_gpu.cuh:25-29,
+ *      src/gpu/solver_gpu.cu:741-782).  Synchronises `stream`.
+ *      dt = cfl*min(dx,dy)/max over interior cells of max(|u|+a, |v|+a);
+ *      MH_ERR_WAVESPEED when that max is <= 0. ---------------------------- */
+int mh_compute_dt(mh_grid g, double cfl, double* dt, double* max_speed, mh_stream stream);
+
+/* ---- advance: replaces advance_second_order_gpu / advance_first_order_gpu
+ *      (head/gpu/solver_gpu.cuh:31-44, src/gpu/solver_gpu.cu:788-885) and
+ *      the CPU twins (head/cpu/solver_cpu.hpp:46-79).
+ *      Second order: x sweep Uold -> Utmp, ghosts of Utmp, y sweep
+ *      Utmp -> Unew, ghosts of Unew (src/cpu/solver_cpu.cpp:426-498).
+ *      Unew may alias Uold (two-buffer stepping); Utmp must be distinct.
+ *      bc NULL = transmissive on every edge. --------------------------------- */
+int mh_advance_second_order(mh_grid Uold, mh_grid Utmp, mh_grid Unew, double dt, int solver, const mh_bc* bc,
+                            mh_stream stream);
+int mh_advance_first_order(mh_grid Uold, mh_grid Unew, double dt, int solver, const mh_bc* bc, mh_stream stream);
+/* One sweep alone (dir 0 = x, 1 = y) over interior cells of `in` -> `out`. */
+int mh_sweep(mh_grid in, mh_grid out, double dt, int dir, int solver, mh_stream stream);
+
+/* ---- device-resident time loop (no host round trip per step).  Replaces the
+ *      loop of scripts/gpu/main_gpu.cu:243-299: each step computes dt on the
+ *      device from the previous step's fused max-signal-speed reduction,
+ *      clamps it to t_end - t, advances, and accumulates t.  --------------- */
+/* Creates a stepper over state U (owned by the caller) with its own scratch
+ * grid.  t_end <= 0: no clamp, steps run unconditionally.  bc NULL =
+ * transmissive.  Fills the ghosts of U and reduces its max signal speed. */
+int mh_stepper_create(mh_grid U, int solver, double cfl, double t_end, const mh_bc* bc, int dt_log_capacity,
+                      mh_stepper* out, mh_stream stream);
+int mh_stepper_destroy(mh_stepper s);
+/* Enqueues n steps (steps past t_end are no-ops).  No synchronisation. */
+int mh_stepper_advance(mh_stepper s, int nsteps, mh_stream stream);
+/* Synchronises and reads the clock: time, completed steps, dt log (up to
+ * capacity entries; may be NULL).  Returns MH_ERR_WAVESPEED if a step met a
+ * non-positive max signal speed. */
+int mh_stepper_clock(mh_stepper s, double* t, int* steps, double* dt_log, mh_stream stream);
+/* Re-fills the ghosts of the state (after the last step). */
+int mh_stepper_finish(mh_stepper s, mh_stream stream);
+/* mh_stepper_advance with a HIP event pair around every launch; synchronises.
+ * ms_total: elapsed time of the n steps; ms_kernels[3]: average duration per
+ * step of the clock kernel, the x sweep and the y sweep.  (No reference
+ * counterpart: the reference times whole runs externally.) */
+int mh_stepper_advance_profiled(mh_stepper s, int nsteps, mh_stream stream, double* ms_total, double* ms_kernels);
+/* Launch shape of the sweeps: x-sweep row segment length (0 = whole row) and
+ * y-sweep rows per wavefront chunk (0 = whole column).  Process-wide; set by
+ * the autotuner.  No reference counterpart (src/gpu/solver_gpu.cu:830-831
+ * hard-codes 16x16 blocks). */
+int mh_set_launch_shape(int seg_x, int seg_y);
+
+/* ---- initial conditions on the device (point sampling at cell centres,
+ *      ghosts included: src/init.cpp:29-38).  kind 0: reference shock_bubble
+ *      (src/test_cases.cpp:24-69); 1: reference blast_wave (:78-101);
+ *      2: blast with p[0..6] = x0,y0,r0,rho_in,p_in,rho_out,p_out;
+ *      3: shock-bubble with p[0..7] = rho0,p0,mach,shock_x,bubble_rho,
+ *      bubble_x,bubble_y,bubble_r. ------------------------------------------ */
+int mh_init_state(mh_grid g, int kind, const double* params, mh_stream stream);
+
+/* ---- pointwise entry points over host arrays of n states (synchronous) ---- */
+/* riemann_flux (head/riemann.hpp:650-672): F = flux(UL, UR) along dir for n
+ * faces of AoS conserved states.  iters (may be NULL) receives the exact
+ * solver's Newton iteration count per face (0 for the other solvers). */
+int mh_riemann_flux(int n, const double* UL, const double* UR, int dir, int solver, double* F, int* iters);
+/* reconstruct_cell_muscl_hancock (src/cpu/solver_cpu.cpp:68-99) for n cells:
+ * evolved left/right edge states from (Um, Uc, Up) and dt/d. */
+int mh_reconstruct(int n, const double* Um, const double* Uc, const double* Up, double dt_over_d, int dir, double* UL,
+                   double* UR);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MH_EULER_H */

@@ -1,0 +1,997 @@
+// solver.hip -- gfx950 kernels of the plain MUSCL-Hancock path and the C ABI
+// over them (include/mh_euler.h).
+//
+// Kernels
+//   k_sweep_y<S>   column-streaming y sweep: each lane owns one column, the
+//                  wavefront covers 64 adjacent columns (row-coalesced loads)
+//                  and marches up a chunk of rows with a register window.
+//                  Each cell is reconstructed once and each face solved once.
+//                  Epilogue: left/right ghost columns of the output and the
+//                  wavefront max of the output's signal speed (one atomic max
+//                  per wavefront) -- the next step's dt needs no extra pass.
+//   k_sweep_x<S>   row-streaming x sweep: the wavefront marches along a row
+//                  segment 64 cells at a time; neighbour values cross lanes
+//                  through LDS with a two-cell lag, again one reconstruction
+//                  per cell and one Riemann solve per face.  Epilogue: bottom/
+//                  top ghost rows of the output (read by the y sweep).
+//   k_max_speed    wavefront-shuffle max of the signal speed (first step).
+//   k_clock        one thread: commits the previous step (t += dt), forms dt
+//                  from the reduced max speed and clamps it to t_end.
+//   k_fill_ghosts  all ghost cells from the boundary spec.
+//   k_first_order<S>, k_init   first-order step, initial conditions.
+// The reference launches 16x16 blocks that reconstruct 18/16 cells and solve
+// 17/16 faces per output cell (src/gpu/solver_gpu.cu:301-683).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "euler_device.h"
+#include "grid.h"
+
+namespace mh {
+
+// ---------------------------------------------------------------------------
+// error plumbing
+// ---------------------------------------------------------------------------
+static thread_local std::string g_err;
+void set_error(const std::string& m) { g_err = m; }
+int fail(int code, const std::string& m) {
+    g_err = m;
+    return code;
+}
+int check_hip(hipError_t e, const char* what) {
+    if (e == hipSuccess) return MH_OK;
+    return fail(e == hipErrorOutOfMemory ? MH_ERR_OOM : MH_ERR_LAUNCH,
+                std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// ---------------------------------------------------------------------------
+// device helpers
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ Cons ld(const View& v, int i, int j) {
+    const double* q = v.o + (int64_t)j * v.rp + i;
+    return {q[0], q[v.ps], q[2 * v.ps], q[3 * v.ps]};
+}
+__device__ __forceinline__ void st(const View& v, int i, int j, const Cons& U) {
+    double* q = v.o + (int64_t)j * v.rp + i;
+    q[0] = U.r;
+    q[v.ps] = U.mu;
+    q[2 * v.ps] = U.mv;
+    q[3 * v.ps] = U.e;
+}
+
+// Boundary spec in kernel-argument form.
+struct Bc {
+    int kind[4];
+    Cons s[4];
+};
+
+// Device clock of the device-resident time loop.
+struct Clock {
+    double t, dt, t_end, cfl, h;
+    unsigned long long smax[2];   // bit patterns of non-negative doubles
+    int step, slot, pending, done, status, log_cap;
+};
+
+// dt source of a sweep: a host value or the device clock.
+struct DtSrc {
+    Clock* clock;
+    double dt;
+};
+
+__device__ __forceinline__ bool dt_of(const DtSrc& s, double& dt) {
+    if (s.clock) {
+        if (s.clock->done) return false;
+        dt = s.clock->dt;
+    } else {
+        dt = s.dt;
+    }
+    return true;
+}
+
+__device__ __forceinline__ double wave_max(double m) {
+    for (int off = 32; off >= 1; off >>= 1) m = smax(m, __shfl_xor(m, off, 64));
+    return m;
+}
+
+// Contribution of one cell's signal speed to a running max with the
+// semantics of std::max(running, s) (NaN speeds never enter).
+__device__ __forceinline__ double run_max(double m, double s) { return smax(m, s); }
+
+__device__ __forceinline__ void atomic_max_pos(unsigned long long* slot, double m) {
+    if (m > 0.0) atomicMax(slot, (unsigned long long)__double_as_longlong(m));
+}
+
+// ---------------------------------------------------------------------------
+// y sweep
+// ---------------------------------------------------------------------------
+struct SweepArgs {
+    DtSrc dts;
+    double d;          // dx or dy
+    int seg;           // rows (y) or cells (x) per wavefront
+    int ngroups;       // column groups (y) or segments per row (x)
+    int ghosts;        // write the fused ghost layers of the output
+    Bc bc;
+    // y sweep only: where the wavefront max of the output's signal speed
+    // goes -- a fixed slot, or (clock_speed) the clock's inactive slot.
+    unsigned long long* speed;
+    int clock_speed;
+};
+
+template <int S>
+__global__ __launch_bounds__(64) void k_sweep_y(View in, View out, SweepArgs a) {
+    double dt;
+    if (!dt_of(a.dts, dt)) return;
+    const double dtd = dt / a.d;
+    const double half = 0.5 * dtd;
+    const int lane = threadIdx.x;
+    const int cg = blockIdx.x % a.ngroups;
+    const int ch = blockIdx.x / a.ngroups;
+    const int i = cg * 64 + lane;
+    const bool active = i < in.nx;
+    const int ic = active ? i : in.nx - 1;
+    const int j0 = ch * a.seg;
+    const int j1 = min(j0 + a.seg, in.ny);
+    int it = 0;
+
+    // p = j0-2, j0-1: load and convert
+    Cons U2 = ld(in, ic, j0 - 2), U1 = ld(in, ic, j0 - 1);
+    Prim W2 = c2p(U2), W1 = c2p(U1);
+    Cons Un = ld(in, ic, j0);
+    // p = j0: reconstruct q = j0-1
+    Edge L, R, Rp;
+    {
+        const Cons U0 = Un;
+        Un = ld(in, ic, j0 + 1);
+        const Prim W0 = c2p(U0);
+        reconstruct<kY>(W2, W1, W0, half, L, Rp);
+        U2 = U1; U1 = U0; W2 = W1; W1 = W0;
+    }
+    // p = j0+1: reconstruct q = j0, flux at face j0-1/2
+    Cons Fp;
+    {
+        const Cons U0 = Un;
+        Un = ld(in, ic, j0 + 2);
+        const Prim W0 = c2p(U0);
+        reconstruct<kY>(W2, W1, W0, half, L, R);
+        Fp = riemann<S, kY>(Rp.U, Rp.V, L.U, L.V, it);
+        Rp = R;
+        U2 = U1; U1 = U0; W2 = W1; W1 = W0;
+    }
+    double m = 0.0;
+    // p = j0+2 .. j1+1: reconstruct q = p-1, face q-1/2, update row r = p-2
+    for (int p = j0 + 2; p <= j1 + 1; ++p) {
+        const Cons U0 = Un;
+        Un = ld(in, ic, min(p + 1, j1 + 1));
+        const Prim W0 = c2p(U0);
+        reconstruct<kY>(W2, W1, W0, half, L, R);
+        const Cons F = riemann<S, kY>(Rp.U, Rp.V, L.U, L.V, it);
+        const int r = p - 2;
+        const Cons Unew = update(U2, dtd, Fp, F);
+        if (active) {
+            st(out, i, r, Unew);
+            if (a.speed || a.clock_speed) m = run_max(m, signal_speed(Unew));
+            if (a.ghosts) {
+                if (i == 0) {
+                    const Cons G = a.bc.kind[MH_EDGE_LEFT] == MH_BC_INFLOW ? a.bc.s[MH_EDGE_LEFT] : Unew;
+                    st(out, -1, r, G);
+                    st(out, -2, r, G);
+                }
+                if (i == in.nx - 1) {
+                    const Cons G = a.bc.kind[MH_EDGE_RIGHT] == MH_BC_INFLOW ? a.bc.s[MH_EDGE_RIGHT] : Unew;
+                    st(out, in.nx, r, G);
+                    st(out, in.nx + 1, r, G);
+                }
+            }
+        }
+        Fp = F;
+        Rp = R;
+        U2 = U1; U1 = U0; W2 = W1; W1 = W0;
+    }
+    if (a.speed || a.clock_speed) {
+        m = wave_max(m);
+        if (lane == 0) {
+            unsigned long long* slot = a.clock_speed ? &a.dts.clock->smax[a.dts.clock->slot ^ 1] : a.speed;
+            atomic_max_pos(slot, m);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// x sweep
+// ---------------------------------------------------------------------------
+// LDS exchange slots: 2 carry slots + 64 lanes.
+constexpr int kSlots = 66;
+
+template <int S>
+__global__ __launch_bounds__(64) void k_sweep_x(View in, View out, SweepArgs a) {
+    double dt;
+    if (!dt_of(a.dts, dt)) return;
+    const double dtd = dt / a.d;
+    const double half = 0.5 * dtd;
+    const int lane = threadIdx.x;
+    const int j = blockIdx.x / a.ngroups;
+    const int sg = blockIdx.x - j * a.ngroups;
+    const int i0 = sg * a.seg;
+    const int i1 = min(i0 + a.seg, in.nx);
+    __shared__ double sW[4][kSlots];
+    __shared__ double sR[8][kSlots];
+    __shared__ double sF[4][kSlots];
+    int it = 0;
+    const int niter = (i1 - i0 + 4 + 63) / 64;
+    Cons Un = ld(in, min(i0 - 2 + lane, in.nx + 1), j);
+    for (int k = 0; k < niter; ++k) {
+        const int p = i0 - 2 + 64 * k + lane;   // cell loaded by this lane
+        const Cons U0 = Un;
+        Un = ld(in, min(p + 64, in.nx + 1), j);
+        const Prim W0 = c2p(U0);
+        // W_{p-1}, W_{p-2}
+        sW[0][2 + lane] = W0.r; sW[1][2 + lane] = W0.u; sW[2][2 + lane] = W0.v; sW[3][2 + lane] = W0.p;
+        __syncthreads();
+        const Prim W1 = {sW[0][1 + lane], sW[1][1 + lane], sW[2][1 + lane], sW[3][1 + lane]};
+        const Prim W2 = {sW[0][lane], sW[1][lane], sW[2][lane], sW[3][lane]};
+        __syncthreads();
+        if (lane >= 62) {
+            sW[0][lane - 62] = W0.r; sW[1][lane - 62] = W0.u; sW[2][lane - 62] = W0.v; sW[3][lane - 62] = W0.p;
+        }
+        // reconstruct q = p-1
+        Edge L, R;
+        reconstruct<kX>(W2, W1, W0, half, L, R);
+        // right edge state of cell q-1
+        sR[0][2 + lane] = R.U.r; sR[1][2 + lane] = R.U.mu; sR[2][2 + lane] = R.U.mv; sR[3][2 + lane] = R.U.e;
+        sR[4][2 + lane] = R.V.r; sR[5][2 + lane] = R.V.u; sR[6][2 + lane] = R.V.v; sR[7][2 + lane] = R.V.p;
+        __syncthreads();
+        const Cons RU = {sR[0][1 + lane], sR[1][1 + lane], sR[2][1 + lane], sR[3][1 + lane]};
+        const Prim RV = {sR[4][1 + lane], sR[5][1 + lane], sR[6][1 + lane], sR[7][1 + lane]};
+        __syncthreads();
+        if (lane == 63) {
+            sR[0][1] = R.U.r; sR[1][1] = R.U.mu; sR[2][1] = R.U.mv; sR[3][1] = R.U.e;
+            sR[4][1] = R.V.r; sR[5][1] = R.V.u; sR[6][1] = R.V.v; sR[7][1] = R.V.p;
+        }
+        // face q-1/2 = r+1/2 with r = p-2
+        const Cons F = riemann<S, kX>(RU, RV, L.U, L.V, it);
+        sF[0][2 + lane] = F.r; sF[1][2 + lane] = F.mu; sF[2][2 + lane] = F.mv; sF[3][2 + lane] = F.e;
+        __syncthreads();
+        const Cons Fm = {sF[0][1 + lane], sF[1][1 + lane], sF[2][1 + lane], sF[3][1 + lane]};
+        __syncthreads();
+        if (lane == 63) { sF[0][1] = F.r; sF[1][1] = F.mu; sF[2][1] = F.mv; sF[3][1] = F.e; }
+        const int r = p - 2;
+        if (r >= i0 && r < i1) {
+            const Cons Unew = update(ld(in, r, j), dtd, Fm, F);
+            st(out, r, j, Unew);
+            if (a.ghosts) {
+                if (j == 0) {
+                    const Cons G = a.bc.kind[MH_EDGE_BOTTOM] == MH_BC_INFLOW ? a.bc.s[MH_EDGE_BOTTOM] : Unew;
+                    st(out, r, -1, G);
+                    st(out, r, -2, G);
+                }
+                if (j == in.ny - 1) {
+                    const Cons G = a.bc.kind[MH_EDGE_TOP] == MH_BC_INFLOW ? a.bc.s[MH_EDGE_TOP] : Unew;
+                    st(out, r, in.ny, G);
+                    st(out, r, in.ny + 1, G);
+                }
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// first-order unsplit Godunov step (src/cpu/solver_cpu.cpp:315-369)
+// ---------------------------------------------------------------------------
+template <int S>
+__global__ __launch_bounds__(256) void k_first_order(View in, View out, double dtdx, double dtdy) {
+    const int i = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int j = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (i >= in.nx || j >= in.ny) return;
+    int it = 0;
+    const Cons Uc = ld(in, i, j), Uw = ld(in, i - 1, j), Ue = ld(in, i + 1, j);
+    const Cons Us = ld(in, i, j - 1), Un = ld(in, i, j + 1);
+    const Prim Vc = c2p(Uc), Vw = c2p(Uw), Ve = c2p(Ue), Vs = c2p(Us), Vn = c2p(Un);
+    const Cons Fxp = riemann<S, kX>(Uc, Vc, Ue, Ve, it);
+    const Cons Fxm = riemann<S, kX>(Uw, Vw, Uc, Vc, it);
+    const Cons Fyp = riemann<S, kY>(Uc, Vc, Un, Vn, it);
+    const Cons Fym = riemann<S, kY>(Us, Vs, Uc, Vc, it);
+    const Cons Ux = update(Uc, dtdx, Fxm, Fxp);
+    st(out, i, j, update(Ux, dtdy, Fym, Fyp));
+}
+
+// ---------------------------------------------------------------------------
+// max signal speed over interior cells -> slot (bit pattern of a double)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_max_speed(View v, unsigned long long* slot) {
+    const int64_t n = (int64_t)v.nx * v.ny;
+    double m = 0.0;
+    for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (int64_t)gridDim.x * blockDim.x) {
+        const int j = (int)(c / v.nx);
+        const int i = (int)(c - (int64_t)j * v.nx);
+        m = run_max(m, signal_speed(ld(v, i, j)));
+    }
+    m = wave_max(m);
+    if ((threadIdx.x & 63) == 0) atomic_max_pos(slot, m);
+}
+
+__global__ void k_zero_u64(unsigned long long* p, int n) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) p[k] = 0ull;
+}
+
+// ---------------------------------------------------------------------------
+// device clock: commit the pending step, then (unless commit_only) form the
+// next dt = cfl*h/smax clamped to t_end - t (scripts/cpu/main_cpu.cpp:221-228).
+// ---------------------------------------------------------------------------
+__global__ void k_clock(Clock* c, double* dt_log, int commit_only) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    if (c->pending) {
+        c->t += c->dt;
+        c->step += 1;
+        c->pending = 0;
+        c->slot ^= 1;
+    }
+    if (commit_only || c->done) return;
+    if (c->t_end > 0.0 && !(c->t < c->t_end)) {
+        c->done = 1;
+        return;
+    }
+    const double m = __longlong_as_double((long long)c->smax[c->slot]);
+    if (!(m > 0.0)) {
+        c->done = 1;
+        c->status = MH_ERR_WAVESPEED;
+        return;
+    }
+    double dt = c->cfl * c->h / m;
+    if (c->t_end > 0.0 && c->t + dt > c->t_end) dt = c->t_end - c->t;
+    c->dt = dt;
+    c->smax[c->slot ^ 1] = 0ull;
+    if (dt_log && c->step < c->log_cap) dt_log[c->step] = dt;
+    c->pending = 1;
+}
+
+// ---------------------------------------------------------------------------
+// ghost fill: every ghost cell from the interior and the boundary spec, with
+// the reference's order (x ghosts of interior rows first, then y ghosts of
+// all columns copying row 0 / ny-1 including its x ghosts:
+// head/cpu/boundary_cpu.hpp:5-36).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ Cons ghost_value(const View& v, const Bc& bc, int i, int j) {
+    if (j < 0 && bc.kind[MH_EDGE_BOTTOM] == MH_BC_INFLOW) return bc.s[MH_EDGE_BOTTOM];
+    if (j >= v.ny && bc.kind[MH_EDGE_TOP] == MH_BC_INFLOW) return bc.s[MH_EDGE_TOP];
+    const int jj = j < 0 ? 0 : (j >= v.ny ? v.ny - 1 : j);
+    if (i < 0 && bc.kind[MH_EDGE_LEFT] == MH_BC_INFLOW) return bc.s[MH_EDGE_LEFT];
+    if (i >= v.nx && bc.kind[MH_EDGE_RIGHT] == MH_BC_INFLOW) return bc.s[MH_EDGE_RIGHT];
+    const int ii = i < 0 ? 0 : (i >= v.nx ? v.nx - 1 : i);
+    return ld(v, ii, jj);
+}
+
+// Thread k < 4*(ny): x ghosts of interior rows; then 4*(nx+4): y ghosts.
+__global__ __launch_bounds__(256) void k_fill_ghosts(View v, Bc bc) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nxg = 4 * v.ny;
+    const int nyg = 4 * (v.nx + 4);
+    if (k < nxg) {
+        const int j = k >> 2, g = k & 3;
+        const int i = g < 2 ? -1 - g : v.nx + (g - 2);
+        st(v, i, j, ghost_value(v, bc, i, j));
+    } else if (k < nxg + nyg) {
+        const int q = k - nxg;
+        const int i = q / 4 - 2, g = q & 3;
+        const int j = g < 2 ? -1 - g : v.ny + (g - 2);
+        st(v, i, j, ghost_value(v, bc, i, j));
+    }
+}
+
+// ---------------------------------------------------------------------------
+// initial conditions (point sampling at every cell centre, src/init.cpp:29-38)
+// ---------------------------------------------------------------------------
+struct InitArgs {
+    int kind;         // 0 blast, 1 shock-bubble
+    double xmin, ymin, dx, dy;
+    double x0, y0, r0;
+    Prim in, out;     // blast: inside / outside; shock-bubble: quiescent / post-shock
+    double shock_x, bubble_rho;
+};
+
+__device__ __forceinline__ Cons make_cons(double rho, double u, double v, double p) {  // src/test_cases.cpp:11-18
+    return {rho, rho * u, rho * v, p / kGm1 + 0.5 * rho * (u * u + v * v)};
+}
+
+__global__ __launch_bounds__(256) void k_init(View v, InitArgs a) {
+    const int64_t tnx = v.nx + 4;
+    const int64_t n = tnx * (v.ny + 4);
+    for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (int64_t)gridDim.x * blockDim.x) {
+        const int j = (int)(c / tnx) - 2;
+        const int i = (int)(c % tnx) - 2;
+        const double x = a.xmin + ((double)i + 0.5) * a.dx;
+        const double y = a.ymin + ((double)j + 0.5) * a.dy;
+        const double ddx = x - a.x0, ddy = y - a.y0;
+        Cons U;
+        if (a.kind == 0) {
+            const double r2 = ddx * ddx + ddy * ddy;
+            U = (r2 <= a.r0 * a.r0) ? make_cons(a.in.r, 0.0, 0.0, a.in.p) : make_cons(a.out.r, 0.0, 0.0, a.out.p);
+        } else {
+            double rho = a.in.r, u = 0.0, w = 0.0, p = a.in.p;
+            if (x < a.shock_x) { rho = a.out.r; u = a.out.u; w = 0.0; p = a.out.p; }
+            if (ddx * ddx + ddy * ddy <= a.r0 * a.r0) rho = a.bubble_rho;
+            U = make_cons(rho, u, w, p);
+        }
+        st(v, i, j, U);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// pointwise evaluation (riemann_flux of head/riemann.hpp:650-672 and the
+// reconstruction of src/cpu/solver_cpu.cpp:68-99) over arrays of states
+// ---------------------------------------------------------------------------
+template <int S, int D>
+__global__ __launch_bounds__(256) void k_face_flux(int n, const double* UL, const double* UR, double* F, int* iters) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const Cons L = {UL[4 * k], UL[4 * k + 1], UL[4 * k + 2], UL[4 * k + 3]};
+    const Cons R = {UR[4 * k], UR[4 * k + 1], UR[4 * k + 2], UR[4 * k + 3]};
+    int it = 0;
+    const Cons f = riemann<S, D>(L, c2p(L), R, c2p(R), it);
+    F[4 * k] = f.r; F[4 * k + 1] = f.mu; F[4 * k + 2] = f.mv; F[4 * k + 3] = f.e;
+    if (iters) iters[k] = it;
+}
+
+template <int D>
+__global__ __launch_bounds__(256) void k_recon(int n, const double* Um, const double* Uc, const double* Up, double half,
+                                               double* L, double* R) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const Cons a = {Um[4 * k], Um[4 * k + 1], Um[4 * k + 2], Um[4 * k + 3]};
+    const Cons b = {Uc[4 * k], Uc[4 * k + 1], Uc[4 * k + 2], Uc[4 * k + 3]};
+    const Cons c = {Up[4 * k], Up[4 * k + 1], Up[4 * k + 2], Up[4 * k + 3]};
+    Edge l, r;
+    reconstruct<D>(c2p(a), c2p(b), c2p(c), half, l, r);
+    L[4 * k] = l.U.r; L[4 * k + 1] = l.U.mu; L[4 * k + 2] = l.U.mv; L[4 * k + 3] = l.U.e;
+    R[4 * k] = r.U.r; R[4 * k + 1] = r.U.mu; R[4 * k + 2] = r.U.mv; R[4 * k + 3] = r.U.e;
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+Bc to_bc(const mh_bc* b) {
+    Bc o;
+    for (int e = 0; e < 4; ++e) {
+        o.kind[e] = b ? b->kind[e] : MH_BC_TRANSMISSIVE;
+        o.s[e] = b ? Cons{b->state[e][0], b->state[e][1], b->state[e][2], b->state[e][3]} : Cons{0, 0, 0, 0};
+    }
+    return o;
+}
+
+int valid_bc(const mh_bc* b) {
+    if (!b) return 1;
+    for (int e = 0; e < 4; ++e)
+        if (b->kind[e] != MH_BC_TRANSMISSIVE && b->kind[e] != MH_BC_INFLOW) return 0;
+    return 1;
+}
+
+// Sweep launch shapes (the autotuner may override them).
+struct Shape {
+    int seg_x = 0;    // 0 = whole row per wavefront
+    int seg_y = 256;  // rows per wavefront chunk
+};
+static Shape g_shape;
+
+int launch_sweep(const mh_grid_s* in, const mh_grid_s* out, int dir, int solver, DtSrc dts, const Bc& bc, int ghosts,
+                 unsigned long long* speed, int clock_speed, hipStream_t st) {
+    SweepArgs a;
+    a.dts = dts;
+    a.bc = bc;
+    a.ghosts = ghosts;
+    a.speed = speed;
+    a.clock_speed = clock_speed;
+    const View vi = in->view(), vo = out->view();
+    if (dir == kX) {
+        a.d = in->dx;
+        a.seg = g_shape.seg_x > 0 ? g_shape.seg_x : in->d.nx;
+        a.ngroups = (in->d.nx + a.seg - 1) / a.seg;
+        const dim3 grid((unsigned)((int64_t)a.ngroups * in->d.ny));
+        switch (solver) {
+            case kHLL: hipLaunchKernelGGL(k_sweep_x<kHLL>, grid, dim3(64), 0, st, vi, vo, a); break;
+            case kHLLC: hipLaunchKernelGGL(k_sweep_x<kHLLC>, grid, dim3(64), 0, st, vi, vo, a); break;
+            case kExact: hipLaunchKernelGGL(k_sweep_x<kExact>, grid, dim3(64), 0, st, vi, vo, a); break;
+            default: hipLaunchKernelGGL(k_sweep_x<kForce>, grid, dim3(64), 0, st, vi, vo, a); break;
+        }
+    } else {
+        a.d = in->dy;
+        a.seg = g_shape.seg_y > 0 ? g_shape.seg_y : in->d.ny;
+        a.ngroups = (in->d.nx + 63) / 64;
+        const int nch = (in->d.ny + a.seg - 1) / a.seg;
+        const dim3 grid((unsigned)((int64_t)a.ngroups * nch));
+        switch (solver) {
+            case kHLL: hipLaunchKernelGGL(k_sweep_y<kHLL>, grid, dim3(64), 0, st, vi, vo, a); break;
+            case kHLLC: hipLaunchKernelGGL(k_sweep_y<kHLLC>, grid, dim3(64), 0, st, vi, vo, a); break;
+            case kExact: hipLaunchKernelGGL(k_sweep_y<kExact>, grid, dim3(64), 0, st, vi, vo, a); break;
+            default: hipLaunchKernelGGL(k_sweep_y<kForce>, grid, dim3(64), 0, st, vi, vo, a); break;
+        }
+    }
+    return check_hip(hipGetLastError(), "sweep launch");
+}
+
+int launch_fill(const mh_grid_s* g, const Bc& bc, hipStream_t st) {
+    const int n = 4 * g->d.ny + 4 * (g->d.nx + 4);
+    hipLaunchKernelGGL(k_fill_ghosts, dim3((n + 255) / 256), dim3(256), 0, st, g->view(), bc);
+    return check_hip(hipGetLastError(), "ghost fill launch");
+}
+
+int launch_max_speed(const mh_grid_s* g, unsigned long long* slot, hipStream_t st) {
+    const int64_t n = (int64_t)g->d.nx * g->d.ny;
+    const int blocks = (int)std::min<int64_t>((n + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_max_speed, dim3(blocks), dim3(256), 0, st, g->view(), slot);
+    return check_hip(hipGetLastError(), "max speed launch");
+}
+
+bool same_shape(const mh_grid_s* a, const mh_grid_s* b) {
+    return a->d.nx == b->d.nx && a->d.ny == b->d.ny && a->dx == b->dx && a->dy == b->dy;
+}
+
+}  // namespace mh
+
+using namespace mh;
+
+struct mh_stepper_s {
+    mh_grid U;
+    mh_grid_s* tmp;
+    int solver;
+    Bc bc;
+    Clock* clock;      // device
+    double* dt_log;    // device
+    int log_cap;
+};
+
+extern "C" {
+
+const char* mh_last_error(void) { return g_err.c_str(); }
+const char* mh_version(void) { return "mh_euler 1.0 gfx950"; }
+
+int mh_set_device(int device) { return check_hip(hipSetDevice(device), "hipSetDevice"); }
+
+int mh_grid_create(const mh_grid_desc* desc, mh_grid* out) {
+    if (!desc || !out) return fail(MH_ERR_INVALID, "mh_grid_create: null argument");
+    if (desc->nx < 1 || desc->ny < 1 || !(desc->x_max > desc->x_min) || !(desc->y_max > desc->y_min))
+        return fail(MH_ERR_INVALID, "mh_grid_create: need nx, ny >= 1 and a non-empty domain");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(MH_ERR_NODEVICE, "no HIP device");
+    mh_grid g = new (std::nothrow) mh_grid_s();
+    if (!g) return fail(MH_ERR_OOM, "host allocation failed");
+    g->d = *desc;
+    g->dx = (desc->x_max - desc->x_min) / (double)desc->nx;   // head/gpu/grid_gpu.cuh:40-41
+    g->dy = (desc->y_max - desc->y_min) / (double)desc->ny;
+    g->rp = round_up((int64_t)kLead + desc->nx + kNg, 16);
+    g->ps = round_up(g->rp * (desc->ny + 2 * kNg), 256);
+    (void)hipGetDevice(&g->device);
+    const hipError_t e = hipMalloc(&g->base, (size_t)(4 * g->ps) * sizeof(double));
+    if (e != hipSuccess) {
+        delete g;
+        return check_hip(e, "hipMalloc(grid)");
+    }
+    const hipError_t e2 = hipMalloc((void**)&g->aux, 8 * sizeof(unsigned long long));
+    if (e2 != hipSuccess) {
+        (void)hipFree(g->base);
+        delete g;
+        return check_hip(e2, "hipMalloc(grid aux)");
+    }
+    *out = g;
+    return MH_OK;
+}
+
+int mh_grid_destroy(mh_grid g) {
+    if (!g) return MH_OK;
+    hipError_t e = hipFree(g->base);
+    const hipError_t e2 = hipFree(g->aux);
+    if (e == hipSuccess) e = e2;
+    delete g;
+    return check_hip(e, "hipFree(grid)");
+}
+
+int mh_grid_info(mh_grid g, mh_grid_desc* desc, double* dx, double* dy, int64_t* pitch, int64_t* plane_stride) {
+    if (!g) return fail(MH_ERR_INVALID, "mh_grid_info: null grid");
+    if (desc) *desc = g->d;
+    if (dx) *dx = g->dx;
+    if (dy) *dy = g->dy;
+    if (pitch) *pitch = g->rp;
+    if (plane_stride) *plane_stride = g->ps;
+    return MH_OK;
+}
+
+int mh_grid_plane_ptr(mh_grid g, int c, double** ptr) {
+    if (!g || c < 0 || c > 3 || !ptr) return fail(MH_ERR_INVALID, "mh_grid_plane_ptr: bad argument");
+    *ptr = g->view().o + (int64_t)c * g->ps;
+    return MH_OK;
+}
+
+static int copy_planes(mh_grid g, double* const host[4], bool upload) {
+    const size_t w = (size_t)(g->d.nx + 4), h = (size_t)(g->d.ny + 4);
+    for (int c = 0; c < 4; ++c) {
+        double* dev = g->base + (int64_t)c * g->ps + (kLead - kNg);
+        const hipError_t e = upload ? hipMemcpy2D(dev, g->rp * sizeof(double), host[c], w * sizeof(double),
+                                                  w * sizeof(double), h, hipMemcpyHostToDevice)
+                                    : hipMemcpy2D(host[c], w * sizeof(double), dev, g->rp * sizeof(double),
+                                                  w * sizeof(double), h, hipMemcpyDeviceToHost);
+        if (e != hipSuccess) return check_hip(e, "hipMemcpy2D");
+    }
+    return MH_OK;
+}
+
+int mh_grid_upload_soa(mh_grid g, const double* const planes[4]) {
+    if (!g || !planes) return fail(MH_ERR_INVALID, "mh_grid_upload_soa: null argument");
+    double* p[4] = {const_cast<double*>(planes[0]), const_cast<double*>(planes[1]), const_cast<double*>(planes[2]),
+                    const_cast<double*>(planes[3])};
+    return copy_planes(g, p, true);
+}
+
+int mh_grid_download_soa(mh_grid g, double* const planes[4]) {
+    if (!g || !planes) return fail(MH_ERR_INVALID, "mh_grid_download_soa: null argument");
+    double* p[4] = {planes[0], planes[1], planes[2], planes[3]};
+    return copy_planes(g, p, false);
+}
+
+int mh_grid_upload_aos(mh_grid g, const double* aos) {
+    if (!g || !aos) return fail(MH_ERR_INVALID, "mh_grid_upload_aos: null argument");
+    const size_t n = (size_t)(g->d.nx + 4) * (g->d.ny + 4);
+    std::vector<double> soa(4 * n);
+    for (size_t k = 0; k < n; ++k)
+        for (int c = 0; c < 4; ++c) soa[c * n + k] = aos[4 * k + c];
+    double* p[4] = {soa.data(), soa.data() + n, soa.data() + 2 * n, soa.data() + 3 * n};
+    return copy_planes(g, p, true);
+}
+
+int mh_grid_download_aos(mh_grid g, double* aos) {
+    if (!g || !aos) return fail(MH_ERR_INVALID, "mh_grid_download_aos: null argument");
+    const size_t n = (size_t)(g->d.nx + 4) * (g->d.ny + 4);
+    std::vector<double> soa(4 * n);
+    double* p[4] = {soa.data(), soa.data() + n, soa.data() + 2 * n, soa.data() + 3 * n};
+    const int rc = copy_planes(g, p, false);
+    if (rc) return rc;
+    for (size_t k = 0; k < n; ++k)
+        for (int c = 0; c < 4; ++c) aos[4 * k + c] = soa[c * n + k];
+    return MH_OK;
+}
+
+static int copy_interior_dev(mh_grid g, double* dense, bool to_grid, hipStream_t st) {
+    const size_t nx = (size_t)g->d.nx, ny = (size_t)g->d.ny;
+    for (int c = 0; c < 4; ++c) {
+        double* dev = g->view().o + (int64_t)c * g->ps;
+        double* d = dense + c * nx * ny;
+        const hipError_t e = to_grid ? hipMemcpy2DAsync(dev, g->rp * sizeof(double), d, nx * sizeof(double),
+                                                        nx * sizeof(double), ny, hipMemcpyDeviceToDevice, st)
+                                     : hipMemcpy2DAsync(d, nx * sizeof(double), dev, g->rp * sizeof(double),
+                                                        nx * sizeof(double), ny, hipMemcpyDeviceToDevice, st);
+        if (e != hipSuccess) return check_hip(e, "hipMemcpy2DAsync");
+    }
+    return MH_OK;
+}
+
+int mh_grid_copy_interior_from_device(mh_grid g, const double* src, mh_stream stream) {
+    if (!g || !src) return fail(MH_ERR_INVALID, "mh_grid_copy_interior_from_device: null argument");
+    return copy_interior_dev(g, const_cast<double*>(src), true, (hipStream_t)stream);
+}
+
+int mh_grid_copy_interior_to_device(mh_grid g, double* dst, mh_stream stream) {
+    if (!g || !dst) return fail(MH_ERR_INVALID, "mh_grid_copy_interior_to_device: null argument");
+    return copy_interior_dev(g, dst, false, (hipStream_t)stream);
+}
+
+int mh_grid_swap(mh_grid a, mh_grid b) {
+    if (!a || !b) return fail(MH_ERR_INVALID, "mh_grid_swap: null grid");
+    if (a->d.nx != b->d.nx || a->d.ny != b->d.ny) return fail(MH_ERR_INVALID, "mh_grid_swap: shape mismatch");
+    std::swap(a->base, b->base);
+    return MH_OK;
+}
+
+int mh_fill_ghosts(mh_grid g, const mh_bc* bc, mh_stream stream) {
+    if (!g) return fail(MH_ERR_INVALID, "mh_fill_ghosts: null grid");
+    if (!valid_bc(bc)) return fail(MH_ERR_INVALID, "mh_fill_ghosts: unknown boundary kind");
+    return launch_fill(g, to_bc(bc), (hipStream_t)stream);
+}
+
+int mh_compute_dt(mh_grid g, double cfl, double* dt, double* max_speed, mh_stream stream) {
+    if (!g) return fail(MH_ERR_INVALID, "mh_compute_dt: null grid");
+    hipStream_t st = (hipStream_t)stream;
+    unsigned long long* slot = g->aux;
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, slot, 1);
+    int rc = launch_max_speed(g, slot, st);
+    unsigned long long bits = 0;
+    if (!rc) rc = check_hip(hipMemcpyAsync(&bits, slot, sizeof(bits), hipMemcpyDeviceToHost, st), "hipMemcpyAsync");
+    if (!rc) rc = check_hip(hipStreamSynchronize(st), "hipStreamSynchronize");
+    if (rc) return rc;
+    double m;
+    std::memcpy(&m, &bits, sizeof(m));
+    if (max_speed) *max_speed = m;
+    if (!(m > 0.0)) return fail(MH_ERR_WAVESPEED, "compute_dt: non-positive maximum wave speed");
+    if (dt) *dt = cfl * std::min(g->dx, g->dy) / m;   // src/cpu/solver_cpu.cpp:308
+    return MH_OK;
+}
+
+int mh_sweep(mh_grid in, mh_grid out, double dt, int dir, int solver, mh_stream stream) {
+    if (!in || !out || in == out) return fail(MH_ERR_INVALID, "mh_sweep: need distinct grids");
+    if (!same_shape(in, out)) return fail(MH_ERR_INVALID, "mh_sweep: shape mismatch");
+    if (dir != 0 && dir != 1) return fail(MH_ERR_INVALID, "mh_sweep: dir must be 0 or 1");
+    if (solver < 0 || solver > 3) return fail(MH_ERR_INVALID, "mh_sweep: unknown solver");
+    return launch_sweep(in, out, dir, solver, DtSrc{nullptr, dt}, to_bc(nullptr), 0, nullptr, 0, (hipStream_t)stream);
+}
+
+int mh_advance_second_order(mh_grid Uold, mh_grid Utmp, mh_grid Unew, double dt, int solver, const mh_bc* bc,
+                            mh_stream stream) {
+    if (!Uold || !Utmp || !Unew) return fail(MH_ERR_INVALID, "mh_advance_second_order: null grid");
+    if (Utmp == Uold || Utmp == Unew) return fail(MH_ERR_INVALID, "mh_advance_second_order: Utmp must be distinct");
+    if (!same_shape(Uold, Utmp) || !same_shape(Uold, Unew))
+        return fail(MH_ERR_INVALID, "mh_advance_second_order: shape mismatch");
+    if (solver < 0 || solver > 3) return fail(MH_ERR_INVALID, "mh_advance_second_order: unknown solver");
+    if (!valid_bc(bc)) return fail(MH_ERR_INVALID, "mh_advance_second_order: unknown boundary kind");
+    hipStream_t st = (hipStream_t)stream;
+    const Bc b = to_bc(bc);
+    int rc = launch_sweep(Uold, Utmp, kX, solver, DtSrc{nullptr, dt}, b, 1, nullptr, 0, st);
+    if (!rc) rc = launch_sweep(Utmp, Unew, kY, solver, DtSrc{nullptr, dt}, b, 1, nullptr, 0, st);
+    if (!rc) rc = launch_fill(Utmp, b, st);
+    if (!rc) rc = launch_fill(Unew, b, st);
+    return rc;
+}
+
+int mh_advance_first_order(mh_grid Uold, mh_grid Unew, double dt, int solver, const mh_bc* bc, mh_stream stream) {
+    if (!Uold || !Unew || Uold == Unew) return fail(MH_ERR_INVALID, "mh_advance_first_order: need distinct grids");
+    if (!same_shape(Uold, Unew)) return fail(MH_ERR_INVALID, "mh_advance_first_order: shape mismatch");
+    if (solver < 0 || solver > 3) return fail(MH_ERR_INVALID, "mh_advance_first_order: unknown solver");
+    if (!valid_bc(bc)) return fail(MH_ERR_INVALID, "mh_advance_first_order: unknown boundary kind");
+    hipStream_t st = (hipStream_t)stream;
+    const double dtdx = dt / Uold->dx, dtdy = dt / Uold->dy;
+    const dim3 grid((Uold->d.nx + 63) / 64, (Uold->d.ny + 3) / 4);
+    const View vi = Uold->view(), vo = Unew->view();
+    switch (solver) {
+        case kHLL: hipLaunchKernelGGL(k_first_order<kHLL>, grid, dim3(256), 0, st, vi, vo, dtdx, dtdy); break;
+        case kHLLC: hipLaunchKernelGGL(k_first_order<kHLLC>, grid, dim3(256), 0, st, vi, vo, dtdx, dtdy); break;
+        case kExact: hipLaunchKernelGGL(k_first_order<kExact>, grid, dim3(256), 0, st, vi, vo, dtdx, dtdy); break;
+        default: hipLaunchKernelGGL(k_first_order<kForce>, grid, dim3(256), 0, st, vi, vo, dtdx, dtdy); break;
+    }
+    int rc = check_hip(hipGetLastError(), "first-order launch");
+    if (!rc) rc = launch_fill(Unew, to_bc(bc), st);
+    return rc;
+}
+
+int mh_stepper_create(mh_grid U, int solver, double cfl, double t_end, const mh_bc* bc, int dt_log_capacity,
+                      mh_stepper* out, mh_stream stream) {
+    if (!U || !out) return fail(MH_ERR_INVALID, "mh_stepper_create: null argument");
+    if (solver < 0 || solver > 3) return fail(MH_ERR_INVALID, "mh_stepper_create: unknown solver");
+    if (!valid_bc(bc)) return fail(MH_ERR_INVALID, "mh_stepper_create: unknown boundary kind");
+    if (!(cfl > 0.0)) return fail(MH_ERR_INVALID, "mh_stepper_create: cfl must be positive");
+    hipStream_t st = (hipStream_t)stream;
+    mh_stepper s = new (std::nothrow) mh_stepper_s();
+    if (!s) return fail(MH_ERR_OOM, "host allocation failed");
+    s->U = U;
+    s->solver = solver;
+    s->bc = to_bc(bc);
+    s->log_cap = dt_log_capacity > 0 ? dt_log_capacity : 0;
+    int rc = mh_grid_create(&U->d, &s->tmp);
+    if (rc) { delete s; return rc; }
+    rc = check_hip(hipMalloc((void**)&s->clock, sizeof(Clock)), "hipMalloc(clock)");
+    s->dt_log = nullptr;
+    if (!rc && s->log_cap) rc = check_hip(hipMalloc((void**)&s->dt_log, sizeof(double) * s->log_cap), "hipMalloc(dt log)");
+    if (rc) { mh_stepper_destroy(s); return rc; }
+    Clock c{};
+    c.t = 0.0; c.dt = 0.0; c.t_end = t_end; c.cfl = cfl; c.h = std::min(U->dx, U->dy);
+    c.smax[0] = c.smax[1] = 0ull;
+    c.step = 0; c.slot = 0; c.pending = 0; c.done = 0; c.status = 0; c.log_cap = s->log_cap;
+    rc = check_hip(hipMemcpyAsync(s->clock, &c, sizeof(c), hipMemcpyHostToDevice, st), "hipMemcpyAsync(clock)");
+    if (!rc) rc = launch_fill(U, s->bc, st);
+    if (!rc) rc = launch_max_speed(U, &s->clock->smax[0], st);
+    if (!rc) rc = check_hip(hipStreamSynchronize(st), "hipStreamSynchronize");
+    if (rc) { mh_stepper_destroy(s); return rc; }
+    *out = s;
+    return MH_OK;
+}
+
+int mh_stepper_destroy(mh_stepper s) {
+    if (!s) return MH_OK;
+    if (s->tmp) mh_grid_destroy(s->tmp);
+    if (s->clock) (void)hipFree(s->clock);
+    if (s->dt_log) (void)hipFree(s->dt_log);
+    delete s;
+    return MH_OK;
+}
+
+int mh_stepper_advance(mh_stepper s, int nsteps, mh_stream stream) {
+    if (!s || nsteps < 0) return fail(MH_ERR_INVALID, "mh_stepper_advance: bad argument");
+    hipStream_t st = (hipStream_t)stream;
+    const DtSrc dts{s->clock, 0.0};
+    for (int k = 0; k < nsteps; ++k) {
+        hipLaunchKernelGGL(k_clock, dim3(1), dim3(64), 0, st, s->clock, s->dt_log, 0);
+        int rc = check_hip(hipGetLastError(), "clock launch");
+        // The y sweep reduces the new state's max signal speed into the
+        // clock's inactive slot, which k_clock has just cleared.
+        if (!rc) rc = launch_sweep(s->U, s->tmp, kX, s->solver, dts, s->bc, 1, nullptr, 0, st);
+        if (!rc) rc = launch_sweep(s->tmp, s->U, kY, s->solver, dts, s->bc, 1, nullptr, 1, st);
+        if (rc) return rc;
+    }
+    return MH_OK;
+}
+
+int mh_stepper_clock(mh_stepper s, double* t, int* steps, double* dt_log, mh_stream stream) {
+    if (!s) return fail(MH_ERR_INVALID, "mh_stepper_clock: null stepper");
+    hipStream_t st = (hipStream_t)stream;
+    hipLaunchKernelGGL(k_clock, dim3(1), dim3(64), 0, st, s->clock, s->dt_log, 1);
+    Clock c;
+    int rc = check_hip(hipGetLastError(), "clock launch");
+    if (!rc) rc = check_hip(hipMemcpyAsync(&c, s->clock, sizeof(c), hipMemcpyDeviceToHost, st), "hipMemcpyAsync");
+    if (!rc && dt_log && s->log_cap)
+        rc = check_hip(hipMemcpyAsync(dt_log, s->dt_log, sizeof(double) * std::min(s->log_cap, std::max(c.step, 0)),
+                                      hipMemcpyDeviceToHost, st),
+                       "hipMemcpyAsync");
+    if (!rc) rc = check_hip(hipStreamSynchronize(st), "hipStreamSynchronize");
+    if (rc) return rc;
+    if (t) *t = c.t;
+    if (steps) *steps = c.step;
+    if (c.status) return fail(c.status, "stepper: non-positive maximum wave speed");
+    return MH_OK;
+}
+
+int mh_stepper_finish(mh_stepper s, mh_stream stream) {
+    if (!s) return fail(MH_ERR_INVALID, "mh_stepper_finish: null stepper");
+    return launch_fill(s->U, s->bc, (hipStream_t)stream);
+}
+
+int mh_init_state(mh_grid g, int kind, const double* p, mh_stream stream) {
+    if (!g) return fail(MH_ERR_INVALID, "mh_init_state: null grid");
+    InitArgs a{};
+    a.xmin = g->d.x_min; a.ymin = g->d.y_min; a.dx = g->dx; a.dy = g->dy;
+    double q[8];
+    if (kind == 0) {            // reference shock_bubble (src/test_cases.cpp:24-69)
+        const double r[8] = {1.29, 1.01325e5, 1.22, 0.005, 0.214, 0.035, 0.0445, 0.025};
+        std::memcpy(q, r, sizeof q);
+        kind = 3;
+    } else if (kind == 1) {     // reference blast_wave (src/test_cases.cpp:78-101)
+        const double r[7] = {0.5, 0.5, 0.1, 1.0, 100.0, 1.0, 1.0};
+        std::memcpy(q, r, 7 * sizeof(double));
+        kind = 2;
+    } else if (kind == 2 || kind == 3) {
+        if (!p) return fail(MH_ERR_INVALID, "mh_init_state: params required");
+        std::memcpy(q, p, (kind == 2 ? 7 : 8) * sizeof(double));
+    } else {
+        return fail(MH_ERR_INVALID, "mh_init_state: unknown kind");
+    }
+    if (kind == 2) {
+        a.kind = 0;
+        a.x0 = q[0]; a.y0 = q[1]; a.r0 = q[2];
+        a.in = {q[3], 0.0, 0.0, q[4]};
+        a.out = {q[5], 0.0, 0.0, q[6]};
+    } else {
+        a.kind = 1;
+        // post-shock state from Rankine-Hugoniot (src/test_cases.cpp:35-47)
+        const double rho0 = q[0], p0 = q[1], mach = q[2];
+        const double a0 = std::sqrt(kGamma * p0 / rho0);
+        const double M2 = mach * mach;
+        const double rho2 = rho0 * ((kGamma + 1.0) * M2) / ((kGamma - 1.0) * M2 + 2.0);
+        const double p2 = p0 * (1.0 + 2.0 * kGamma / (kGamma + 1.0) * (M2 - 1.0));
+        const double u2 = 2.0 * a0 / (kGamma + 1.0) * (mach - 1.0 / mach);
+        a.in = {rho0, 0.0, 0.0, p0};
+        a.out = {rho2, u2, 0.0, p2};
+        a.shock_x = q[3];
+        a.bubble_rho = q[4];
+        a.x0 = q[5]; a.y0 = q[6]; a.r0 = q[7];
+    }
+    const int64_t n = (int64_t)(g->d.nx + 4) * (g->d.ny + 4);
+    const int blocks = (int)std::min<int64_t>((n + 255) / 256, 8192);
+    hipLaunchKernelGGL(k_init, dim3(blocks), dim3(256), 0, (hipStream_t)stream, g->view(), a);
+    return check_hip(hipGetLastError(), "init launch");
+}
+
+// Device scratch for the pointwise entry points: freed before returning.
+struct DevBuf {
+    void* p = nullptr;
+    ~DevBuf() { if (p) (void)hipFree(p); }
+};
+
+int mh_riemann_flux(int n, const double* UL, const double* UR, int dir, int solver, double* F, int* iters) {
+    if (n < 0 || (n > 0 && (!UL || !UR || !F))) return fail(MH_ERR_INVALID, "mh_riemann_flux: bad argument");
+    if (dir != 0 && dir != 1) return fail(MH_ERR_INVALID, "mh_riemann_flux: dir must be 0 or 1");
+    if (solver < 0 || solver > 3) return fail(MH_ERR_INVALID, "mh_riemann_flux: unknown solver");
+    if (n == 0) return MH_OK;
+    const size_t bytes = sizeof(double) * 4 * (size_t)n;
+    DevBuf b;
+    int rc = check_hip(hipMalloc(&b.p, 3 * bytes + sizeof(int) * (size_t)n), "hipMalloc");
+    if (rc) return rc;
+    double* dL = (double*)b.p;
+    double* dR = dL + 4 * (size_t)n;
+    double* dF = dR + 4 * (size_t)n;
+    int* dI = (int*)(dF + 4 * (size_t)n);
+    rc = check_hip(hipMemcpy(dL, UL, bytes, hipMemcpyHostToDevice), "hipMemcpy");
+    if (!rc) rc = check_hip(hipMemcpy(dR, UR, bytes, hipMemcpyHostToDevice), "hipMemcpy");
+    if (rc) return rc;
+    const dim3 g((n + 255) / 256), blk(256);
+#define MH_FF(S_, D_) hipLaunchKernelGGL((k_face_flux<S_, D_>), g, blk, 0, 0, n, dL, dR, dF, dI)
+    if (dir == 0) {
+        switch (solver) { case 0: MH_FF(kHLL, kX); break; case 1: MH_FF(kHLLC, kX); break;
+                          case 2: MH_FF(kExact, kX); break; default: MH_FF(kForce, kX); }
+    } else {
+        switch (solver) { case 0: MH_FF(kHLL, kY); break; case 1: MH_FF(kHLLC, kY); break;
+                          case 2: MH_FF(kExact, kY); break; default: MH_FF(kForce, kY); }
+    }
+#undef MH_FF
+    rc = check_hip(hipGetLastError(), "face flux launch");
+    if (!rc) rc = check_hip(hipMemcpy(F, dF, bytes, hipMemcpyDeviceToHost), "hipMemcpy");
+    if (!rc && iters) rc = check_hip(hipMemcpy(iters, dI, sizeof(int) * (size_t)n, hipMemcpyDeviceToHost), "hipMemcpy");
+    return rc;
+}
+
+int mh_reconstruct(int n, const double* Um, const double* Uc, const double* Up, double dt_over_d, int dir, double* UL,
+                   double* UR) {
+    if (n < 0 || (n > 0 && (!Um || !Uc || !Up || !UL || !UR))) return fail(MH_ERR_INVALID, "mh_reconstruct: bad argument");
+    if (dir != 0 && dir != 1) return fail(MH_ERR_INVALID, "mh_reconstruct: dir must be 0 or 1");
+    if (n == 0) return MH_OK;
+    const size_t bytes = sizeof(double) * 4 * (size_t)n;
+    DevBuf b;
+    int rc = check_hip(hipMalloc(&b.p, 5 * bytes), "hipMalloc");
+    if (rc) return rc;
+    double* d = (double*)b.p;
+    const size_t q = 4 * (size_t)n;
+    rc = check_hip(hipMemcpy(d, Um, bytes, hipMemcpyHostToDevice), "hipMemcpy");
+    if (!rc) rc = check_hip(hipMemcpy(d + q, Uc, bytes, hipMemcpyHostToDevice), "hipMemcpy");
+    if (!rc) rc = check_hip(hipMemcpy(d + 2 * q, Up, bytes, hipMemcpyHostToDevice), "hipMemcpy");
+    if (rc) return rc;
+    const double half = 0.5 * dt_over_d;
+    const dim3 g((n + 255) / 256), blk(256);
+    if (dir == 0) hipLaunchKernelGGL(k_recon<kX>, g, blk, 0, 0, n, d, d + q, d + 2 * q, half, d + 3 * q, d + 4 * q);
+    else hipLaunchKernelGGL(k_recon<kY>, g, blk, 0, 0, n, d, d + q, d + 2 * q, half, d + 3 * q, d + 4 * q);
+    rc = check_hip(hipGetLastError(), "reconstruct launch");
+    if (!rc) rc = check_hip(hipMemcpy(UL, d + 3 * q, bytes, hipMemcpyDeviceToHost), "hipMemcpy");
+    if (!rc) rc = check_hip(hipMemcpy(UR, d + 4 * q, bytes, hipMemcpyDeviceToHost), "hipMemcpy");
+    return rc;
+}
+
+int mh_stepper_advance_profiled(mh_stepper s, int nsteps, mh_stream stream, double* ms_total, double* ms_kernels) {
+    if (!s || nsteps < 1) return fail(MH_ERR_INVALID, "mh_stepper_advance_profiled: bad argument");
+    hipStream_t st = (hipStream_t)stream;
+    const DtSrc dts{s->clock, 0.0};
+    // events: [k][0..3] = before clock, before x, before y, after y
+    std::vector<hipEvent_t> ev((size_t)nsteps * 4);
+    int rc = MH_OK;
+    for (auto& e : ev)
+        if (!rc) rc = check_hip(hipEventCreate(&e), "hipEventCreate");
+    for (int k = 0; k < nsteps && !rc; ++k) {
+        hipEvent_t* e = &ev[(size_t)k * 4];
+        rc = check_hip(hipEventRecord(e[0], st), "hipEventRecord");
+        hipLaunchKernelGGL(k_clock, dim3(1), dim3(64), 0, st, s->clock, s->dt_log, 0);
+        if (!rc) rc = check_hip(hipGetLastError(), "clock launch");
+        if (!rc) rc = check_hip(hipEventRecord(e[1], st), "hipEventRecord");
+        if (!rc) rc = launch_sweep(s->U, s->tmp, kX, s->solver, dts, s->bc, 1, nullptr, 0, st);
+        if (!rc) rc = check_hip(hipEventRecord(e[2], st), "hipEventRecord");
+        if (!rc) rc = launch_sweep(s->tmp, s->U, kY, s->solver, dts, s->bc, 1, nullptr, 1, st);
+        if (!rc) rc = check_hip(hipEventRecord(e[3], st), "hipEventRecord");
+    }
+    if (!rc) rc = check_hip(hipStreamSynchronize(st), "hipStreamSynchronize");
+    double sum[3] = {0, 0, 0};
+    for (int k = 0; k < nsteps && !rc; ++k) {
+        hipEvent_t* e = &ev[(size_t)k * 4];
+        for (int q = 0; q < 3 && !rc; ++q) {
+            float ms = 0.f;
+            rc = check_hip(hipEventElapsedTime(&ms, e[q], e[q + 1]), "hipEventElapsedTime");
+            sum[q] += ms;
+        }
+    }
+    float tot = 0.f;
+    if (!rc) rc = check_hip(hipEventElapsedTime(&tot, ev[0], ev[(size_t)nsteps * 4 - 1]), "hipEventElapsedTime");
+    for (auto& e : ev)
+        if (e) (void)hipEventDestroy(e);
+    if (rc) return rc;
+    if (ms_total) *ms_total = tot;
+    if (ms_kernels)
+        for (int q = 0; q < 3; ++q) ms_kernels[q] = sum[q] / nsteps;
+    return MH_OK;
+}
+
+int mh_set_launch_shape(int seg_x, int seg_y) {
+    if (seg_x < 0 || seg_y < 0) return fail(MH_ERR_INVALID, "mh_set_launch_shape: negative segment");
+    g_shape.seg_x = seg_x;
+    g_shape.seg_y = seg_y;
+    return MH_OK;
+}
+
+}  // extern "C"
+
+

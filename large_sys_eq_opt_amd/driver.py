@@ -1,0 +1,141 @@
+"""Time loop, snapshot schedule and snapshot output.
+
+``run_case`` is the loop of scripts/gpu/main_gpu.cu:243-299 and
+scripts/cpu/main_cpu.cpp:206-262: while t < t_end: dt = compute_dt (the
+boundary fill is fused into the previous step), dt = min(dt, t_end - t),
+advance, swap, t += dt; a snapshot is written at the first step with
+t >= k*t_end/num_snapshots.  ``mode="host"`` computes every dt with a host
+round trip exactly as the reference does; ``mode="device"`` keeps dt and t
+on the device (same arithmetic, one synchronisation at the end) and cannot
+write intermediate snapshots.
+
+Snapshot CSV: header ``i,j,x,y,rho,rhou,rhov,E``, one row per interior cell,
+i and j including the ghost offset, 6 significant digits
+(scripts/cpu/main_cpu.cpp:119-164); ``precision=17`` writes round-trip
+exact values instead.
+"""
+from __future__ import annotations
+
+import os
+import time
+
+import numpy as np
+
+from . import cases as _cases
+from .solver import (Boundary, Grid2DGPU, Stepper, advance_first_order_gpu, advance_second_order_gpu,
+                     compute_dt_gpu, parse_riemann_solver, riemann_solver_to_string)
+
+
+def snapshot_name(prefix: str, case_name: str, solver_name: str, scale: int, k: int, outdir="outputs") -> str:
+    return os.path.join(outdir, f"{prefix}_{case_name}_{solver_name}_n{scale}_snapshot_{k}.csv")
+
+
+def _fmt(v: float, precision: int) -> str:
+    return format(float(v), f".{precision}g")
+
+
+def write_grid_csv(U_aos: np.ndarray, cfg, filename: str, precision: int = 6) -> None:
+    """scripts/cpu/main_cpu.cpp:136-164 (ostream default = 6 significant digits)."""
+    ng = cfg.ng
+    dx = (cfg.x_max - cfg.x_min) / float(cfg.nx)
+    dy = (cfg.y_max - cfg.y_min) / float(cfg.ny)
+    os.makedirs(os.path.dirname(filename) or ".", exist_ok=True)
+    with open(filename, "w") as f:
+        f.write("i,j,x,y,rho,rhou,rhov,E\n")
+        for j in range(ng, ng + cfg.ny):
+            y = cfg.y_min + (float(j - ng) + 0.5) * dy
+            row = U_aos[j]
+            for i in range(ng, ng + cfg.nx):
+                x = cfg.x_min + (float(i - ng) + 0.5) * dx
+                U = row[i]
+                f.write(f"{i},{j},{_fmt(x, precision)},{_fmt(y, precision)},{_fmt(U[0], precision)},"
+                        f"{_fmt(U[1], precision)},{_fmt(U[2], precision)},{_fmt(U[3], precision)}\n")
+
+
+def read_grid_csv(filename: str, nx: int, ny: int) -> np.ndarray:
+    """Reads a snapshot CSV back into an interior array of shape (ny, nx, 4)."""
+    d = np.loadtxt(filename, delimiter=",", skiprows=1)
+    out = np.zeros((ny, nx, 4))
+    out[d[:, 1].astype(int) - 2, d[:, 0].astype(int) - 2] = d[:, 4:8]
+    return out
+
+
+def run_case(case, solver, scale: int = 1, *, mode: str = "host", num_snapshots: int = 0, write_output=False,
+             outdir="outputs", precision=6, order: int = 2, max_steps: int | None = None, t_end: float | None = None,
+             bc: Boundary | None = None, U0: np.ndarray | None = None, keep_snapshots=False, log=None):
+    """Runs a case through the HIP solver; returns dict(U, t, steps, dt, snap_t, seconds)."""
+    case_id = _cases.parse_case_id(case)
+    s = parse_riemann_solver(solver)
+    cfg = _cases.get_n_case_config(case_id, scale)
+    if t_end is not None:
+        cfg.t_end = t_end
+    bc = _cases.case_boundary(case_id) if bc is None else bc
+    U = Grid2DGPU.from_config(cfg)
+    if U0 is None:
+        kind, params = _cases.case_params(case_id)
+        U.init_state(kind, params.array())
+    else:
+        U.upload_from_aos(U0)
+    if max_steps is None and not cfg.t_end > 0:
+        raise ValueError("run_case: give max_steps when the case has no t_end")
+    limit = max_steps if max_steps is not None else (1 << 62)
+    snap_times = [cfg.t_end * float(k) / float(num_snapshots) for k in range(1, num_snapshots + 1)]
+    snaps, snap_t, dts = [], [], []
+    t, step = 0.0, 0
+    t0 = time.perf_counter()
+    if mode == "device":
+        if num_snapshots and (write_output or keep_snapshots):
+            raise ValueError("mode='device' does not take intermediate snapshots; use mode='host'")
+        cap = int(min(limit, 1 << 22)) if max_steps is not None else 1 << 20
+        st = Stepper(U, s, cfg.cfl, cfg.t_end if max_steps is None or t_end is not None else 0.0, bc,
+                     log_capacity=cap)
+        n = limit if max_steps is not None else None
+        if n is None:
+            while True:   # batches of steps until the clock passes t_end
+                st.advance(256)
+                t, step, _ = st.clock()
+                if t >= cfg.t_end:
+                    break
+        else:
+            st.advance(int(n))
+        t, step, dts = st.clock()
+        st.finish()
+        st.close()
+    else:
+        tmp, nxt = Grid2DGPU.from_config(cfg), Grid2DGPU.from_config(cfg)
+        from .solver import apply_boundary_gpu
+
+        apply_boundary_gpu(U, bc)
+        nk = 0
+        while (cfg.t_end > 0 and t < cfg.t_end or cfg.t_end <= 0) and step < limit:
+            dt = compute_dt_gpu(U, None, cfg.cfl)
+            if cfg.t_end > 0 and t + dt > cfg.t_end:
+                dt = cfg.t_end - t
+            if order == 1:
+                advance_first_order_gpu(U, nxt, dt, s, bc)
+            else:
+                advance_second_order_gpu(U, tmp, nxt, None, dt, s, bc)
+            U.swap(nxt)
+            t += dt
+            step += 1
+            dts.append(dt)
+            while nk < num_snapshots and t >= snap_times[nk]:
+                if write_output or keep_snapshots:
+                    A = U.download_to_aos()
+                    if keep_snapshots:
+                        snaps.append(A)
+                    if write_output:
+                        fn = snapshot_name("gpu", _cases.case_id_to_string(case_id), riemann_solver_to_string(s),
+                                           scale, nk + 1, outdir)
+                        write_grid_csv(A, cfg, fn, precision)
+                        if log:
+                            log(f"[GPU] Wrote snapshot {nk + 1} at t = {t:.6g} -> {fn}")
+                snap_t.append(t)
+                nk += 1
+        tmp.close()
+        nxt.close()
+    seconds = time.perf_counter() - t0
+    out = dict(U=U.download_to_aos(), t=t, steps=step, dt=np.asarray(dts), snap_t=np.asarray(snap_t),
+               snapshots=snaps, seconds=seconds, cfg=cfg)
+    U.close()
+    return out

@@ -1,0 +1,72 @@
+"""Shared fixtures.  GPU tests are marked ``gpu``; everything else runs on the CPU.
+
+The oracle (oracle/_build/liboracle.so) and, where the reference tree was
+present at build time, the compiled reference (oracle/_ref/libmuscl_ref.so)
+are the checkers.  The product under test is libmh_euler.so.
+"""
+from __future__ import annotations
+
+import os
+import sys
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+
+import pyoracle  # noqa: E402
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an AMD GPU (MI355X) and libmh_euler.so")
+    config.addinivalue_line("markers", "slow: long CPU run (full reference step counts)")
+
+
+@pytest.fixture(scope="session")
+def oracle():
+    if not os.path.exists(pyoracle.ORACLE_SO):
+        pyoracle.build(reference=False)
+    return pyoracle.Oracle()
+
+
+@pytest.fixture(scope="session")
+def reference():
+    if not pyoracle.reference_available():
+        pytest.skip("reference harness not built (needs the reference tree at build time)")
+    return pyoracle.Reference()
+
+
+@pytest.fixture(scope="session")
+def hip():
+    """The HIP library; fails (does not skip) when it cannot be loaded."""
+    import large_sys_eq_opt_amd as m
+
+    m.load_library()
+    return m
+
+
+def interior(U):
+    return U[2:-2, 2:-2]
+
+
+def rel_linf(a, b):
+    """max |a-b| / max |b| per conserved component, maximised."""
+    a, b = interior(a), interior(b)
+    out = 0.0
+    for c in range(4):
+        den = np.abs(b[..., c]).max()
+        num = np.abs(a[..., c] - b[..., c]).max()
+        out = max(out, num / den if den > 0 else num)
+    return out
+
+
+def random_states(rng, n, rho=(0.1, 2.0), vel=(-1.5, 1.5), p=(0.05, 5.0)):
+    """n random physical primitive states (rho, u, v, p)."""
+    V = np.empty((n, 4))
+    V[:, 0] = rng.uniform(*rho, n)
+    V[:, 1] = rng.uniform(*vel, n)
+    V[:, 2] = rng.uniform(*vel, n)
+    V[:, 3] = rng.uniform(*p, n)
+    return V

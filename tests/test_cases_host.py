@@ -1,0 +1,70 @@
+"""Host-side case setup, solver selection and snapshot I/O (no GPU)."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from large_sys_eq_opt_amd import cases
+from large_sys_eq_opt_amd.driver import read_grid_csv, snapshot_name, write_grid_csv
+from large_sys_eq_opt_amd.solver import parse_riemann_solver, riemann_solver_to_string
+
+
+def test_case_configs_match_reference():
+    c = cases.get_case_config("shock_bubble")   # src/test_cases.cpp:134-145
+    assert (c.nx, c.ny, c.ng, c.x_max, c.y_max, c.cfl, c.t_end) == (500, 197, 2, 0.225, 0.089, 0.4, 0.0011741)
+    c = cases.get_case_config("blast_wave")     # src/test_cases.cpp:147-158
+    assert (c.nx, c.ny, c.x_max, c.cfl, c.t_end) == (500, 500, 1.0, 0.4, 0.2)
+    c = cases.get_n_case_config("cyl_blast", 32)
+    assert (c.nx, c.ny, c.x_max, c.cfl) == (8192, 8192, 2.0, 0.8)
+    with pytest.raises(ValueError):
+        cases.get_n_case_config("blast_wave", 0)
+    with pytest.raises(ValueError):
+        cases.parse_case_id("no_such_case")
+
+
+def test_post_shock_state():
+    # Mach 1.22 into (rho, p) = (1, 1): values quoted for the nondimensional case
+    rho2, u2, p2 = cases.post_shock_state(1.0, 1.0, 1.22)
+    assert abs(rho2 - 1.376364) < 1e-6 and abs(u2 - 0.394729) < 1e-6 and abs(p2 - 1.5698) < 1e-4
+    # Rankine-Hugoniot: mass and momentum flux continuity in the shock frame
+    s = 1.22 * np.sqrt(1.4)
+    assert abs(1.0 * s - rho2 * (s - u2)) < 1e-12
+    assert abs(1.0 * s * s + 1.0 - (rho2 * (s - u2) ** 2 + p2)) < 1e-12
+
+
+@pytest.mark.parametrize("name", ["shock_bubble", "blast_wave", "cyl_blast", "shock_bubble_nd"])
+def test_host_initial_state_matches_oracle(oracle, name):
+    cfg = cases.get_case_config(name)
+    kind, params = cases.case_params(name)
+    assert np.array_equal(cases.initialise_grid(name, cfg), oracle.init(kind, params.array(), cfg.nx, cfg.ny,
+                                                                           cfg.bounds))
+
+
+def test_reference_initial_grid(reference):
+    for case_id, name in ((0, "shock_bubble"), (1, "blast_wave")):
+        U, _ = reference.init_case(case_id, 1)
+        mine, _ = cases.make_n_grid(name, 1)
+        assert np.array_equal(U, mine)
+
+
+def test_solver_names():
+    assert parse_riemann_solver("hllc") == 1 and parse_riemann_solver("EXACT") == 2
+    assert riemann_solver_to_string(3) == "force"
+    with pytest.raises(ValueError):
+        parse_riemann_solver("roe")
+
+
+def test_snapshot_csv_format_and_roundtrip(tmp_path):
+    cfg = cases.get_case_config("cyl_blast")
+    cfg.nx, cfg.ny = 6, 4
+    U = cases.initialise_grid("cyl_blast", cfg)
+    fn = str(tmp_path / "s.csv")
+    write_grid_csv(U, cfg, fn)
+    lines = open(fn).read().splitlines()
+    assert lines[0] == "i,j,x,y,rho,rhou,rhov,E"           # scripts/cpu/main_cpu.cpp:140
+    assert lines[1].startswith("2,2,0.166667,0.25,")        # indices include ng; 6 significant digits
+    assert len(lines) == 1 + 24
+    exact = str(tmp_path / "e.csv")
+    write_grid_csv(U, cfg, exact, precision=17)
+    assert np.array_equal(read_grid_csv(exact, 6, 4), U[2:-2, 2:-2])
+    assert snapshot_name("gpu", "blast_wave", "hllc", 2, 3) == "outputs/gpu_blast_wave_hllc_n2_snapshot_3.csv"
