@@ -156,6 +156,33 @@ int mh_set_launch_shape(int seg_x, int seg_y);
  *      bubble_x,bubble_y,bubble_r. ------------------------------------------ */
 int mh_init_state(mh_grid g, int kind, const double* params, mh_stream stream);
 
+/* ---- batched ensemble (no reference counterpart): n independent grids of
+ *      uneven size, each with its own CFL dt, advanced one split step per
+ *      persistent launch; step state lives on the device, so a captured
+ *      graph of k launches replays k steps. -------------------------------- */
+/* Allocates n instance grids (descs[k]) and their scratch in one arena.
+ * bcs: n boundary specs or NULL (transmissive). */
+int mh_batch_create(int n, const mh_grid_desc* descs, int solver, double cfl, const mh_bc* bcs, mh_batch* out);
+int mh_batch_destroy(mh_batch b);
+/* Non-owning handle of instance k: upload/download/init/fill through the
+ * grid entry points (mh_grid_destroy refuses it). */
+int mh_batch_grid(mh_batch b, int k, mh_grid* g);
+/* After the instances are initialised: fills ghosts, resets the clocks and
+ * reduces every instance's initial max signal speed.  Synchronises. */
+int mh_batch_prepare(mh_batch b, mh_stream stream);
+/* Launch schedule: x tile row segment (0 = whole row), y tile chunk rows
+ * (0 = whole column), two launches per step instead of one fused launch,
+ * workgroups per CU (0 = occupancy limit).  Drops a captured graph. */
+int mh_batch_set_schedule(mh_batch b, int xseg, int ychunk, int two_launch, int blocks_per_cu);
+/* Enqueues n steps of every instance (graph replays where captured). */
+int mh_batch_step(mh_batch b, int nsteps, mh_stream stream);
+/* Captures steps_per_graph steps into a hipGraph used by mh_batch_step. */
+int mh_batch_capture(mh_batch b, int steps_per_graph, mh_stream stream);
+/* Synchronises; per-instance time, last dt, steps, status (arrays of n or NULL). */
+int mh_batch_clock(mh_batch b, double* t, double* dt, int* steps, int* status, mh_stream stream);
+/* Persistent grid size and tile counts of the current schedule. */
+int mh_batch_info(mh_batch b, int* nblocks, int* nxtiles, int* nytiles);
+
 /* ---- pointwise entry points over host arrays of n states (synchronous) ---- */
 /* riemann_flux (head/riemann.hpp:650-672): F = flux(UL, UR) along dir for n
  * faces of AoS conserved states.  iters (may be NULL) receives the exact

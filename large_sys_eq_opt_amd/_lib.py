@@ -61,6 +61,15 @@ SIGNATURES = {
     "mh_init_state": ([_P, _I, _P, _P], _I),
     "mh_stepper_advance_profiled": ([_P, _I, _P, C.POINTER(_D), C.POINTER(_D)], _I),
     "mh_set_launch_shape": ([_I, _I], _I),
+    "mh_batch_create": ([_I, _P, _I, _D, _P, C.POINTER(_P)], _I),
+    "mh_batch_destroy": ([_P], _I),
+    "mh_batch_grid": ([_P, _I, C.POINTER(_P)], _I),
+    "mh_batch_prepare": ([_P, _P], _I),
+    "mh_batch_set_schedule": ([_P, _I, _I, _I, _I], _I),
+    "mh_batch_step": ([_P, _I, _P], _I),
+    "mh_batch_capture": ([_P, _I, _P], _I),
+    "mh_batch_clock": ([_P, _P, _P, _P, _P, _P], _I),
+    "mh_batch_info": ([_P, C.POINTER(_I), C.POINTER(_I), C.POINTER(_I)], _I),
     "mh_riemann_flux": ([_I, _P, _P, _I, _I, _P, _P], _I),
     "mh_reconstruct": ([_I, _P, _P, _P, _D, _I, _P, _P], _I),
 }

@@ -1,0 +1,505 @@
+// batch.hip -- a persistent kernel that advances an ensemble of independent
+// grids of uneven size, each with its own CFL dt, by one split step per
+// launch (no reference counterpart: the reference runs one grid per process).
+//
+// Layout: every instance is an ordinary grid (four pitched FP64 planes with
+// ghosts) carved out of one arena, plus a scratch grid for the x-sweep
+// output.  Work is cut into the same wavefront tiles as the single-grid
+// sweeps (sweep_tiles.h): x tiles are row segments, y tiles are 64-column x
+// chunk-row blocks.  The host lays the tiles out instance by instance and
+// splits each phase's list into contiguous ranges of equal estimated cost,
+// one range per wavefront of the persistent grid, so no tile queue (and no
+// cross-XCD atomic per tile) is needed.
+//
+// One launch = one step of every instance:
+//   phase x   each wavefront runs its x tiles with its instance's dt, formed
+//             from the max signal speed reduced in the previous step;
+//             block 0 commits the clocks (t += dt) and clears the slots the
+//             y phase will fill.
+//   barrier   grid-wide (all workgroups are resident: the grid is sized from
+//             the occupancy of this kernel); bounded spin, flags an error
+//             instead of hanging.
+//   phase y   y tiles; each wavefront max of the new signal speed goes to its
+//             instance's slot with one atomic max.
+// Step parity, counters and clocks live in device memory, so the launch is
+// identical every step and a captured graph of k launches replays k steps.
+// The schedule can instead use two launches per step (x, then y): the
+// autotuner times both.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "euler_device.h"
+#include "grid.h"
+#include "sweep_tiles.h"
+
+namespace mh {
+
+struct BatchInst {
+    View U, T;
+    double dx, dy, cfl, h;
+    Bc bc;
+};
+
+struct BatchClock {
+    unsigned long long smax[2];
+    double t, dt;
+    int step, status;
+};
+
+struct XTile { int inst, j, i0, i1; };
+struct YTile { int inst, cg, j0, j1; };
+
+struct Ctrl {
+    int step;
+    unsigned bar[2];
+    unsigned done[2];
+    int err;
+};
+
+struct BatchArgs {
+    const BatchInst* inst;
+    BatchClock* clk;
+    int ninst;
+    const XTile* xt;
+    const int* xstart;   // nwaves + 1
+    const YTile* yt;
+    const int* ystart;   // nwaves + 1
+    Ctrl* ctrl;
+    int phases;          // 1 = x only, 2 = y only, 3 = x, barrier, y
+};
+
+__device__ __forceinline__ bool inst_dt(const BatchInst& I, const BatchClock& c, int par, double& dt) {
+    const double m = __longlong_as_double((long long)c.smax[par]);
+    if (!(m > 0.0)) return false;
+    dt = I.cfl * I.h / m;   // src/cpu/solver_cpu.cpp:308
+    return true;
+}
+
+__device__ void grid_barrier(unsigned* bar, unsigned nblocks, int* err) {
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        atomicAdd(bar, 1u);
+        long spins = 0;
+        while (__hip_atomic_load(bar, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < nblocks) {
+            __builtin_amdgcn_s_sleep(8);
+            if (++spins > (1L << 24)) {   // ~ seconds: not all workgroups resident
+                atomicExch(err, 1);
+                break;
+            }
+        }
+        __threadfence();
+    }
+    __syncthreads();
+}
+
+template <int S>
+__global__ __launch_bounds__(256) void k_batch_step(BatchArgs a) {
+    __shared__ XLds X[4];
+    const int w = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
+    const int gw = blockIdx.x * 4 + w;
+    const int step = __hip_atomic_load(&a.ctrl->step, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const int par = step & 1;
+    if (a.phases & 1) {
+        if (blockIdx.x == 0) {
+            if (threadIdx.x == 0) {
+                a.ctrl->bar[par ^ 1] = 0u;
+                a.ctrl->done[par ^ 1] = 0u;
+            }
+            for (int k = threadIdx.x; k < a.ninst; k += blockDim.x) {
+                BatchClock& c = a.clk[k];
+                double dt;
+                if (inst_dt(a.inst[k], c, par, dt)) {
+                    c.dt = dt;
+                    c.t += dt;
+                    c.step += 1;
+                } else {
+                    c.status = MH_ERR_WAVESPEED;
+                }
+                c.smax[par ^ 1] = 0ull;
+            }
+        }
+        for (int q = a.xstart[gw]; q < a.xstart[gw + 1]; ++q) {
+            const XTile t = a.xt[q];
+            const BatchInst& I = a.inst[t.inst];
+            double dt;
+            if (!inst_dt(I, a.clk[t.inst], par, dt)) continue;
+            const double dtd = dt / I.dx;
+            x_tile<S>(I.U, I.T, t.j, t.i0, t.i1, dtd, 0.5 * dtd, I.bc, 1, X[w], lane);
+        }
+    }
+    if (a.phases == 3) grid_barrier(&a.ctrl->bar[par], gridDim.x, &a.ctrl->err);
+    if (a.phases & 2) {
+        for (int q = a.ystart[gw]; q < a.ystart[gw + 1]; ++q) {
+            const YTile t = a.yt[q];
+            const BatchInst& I = a.inst[t.inst];
+            double dt;
+            if (!inst_dt(I, a.clk[t.inst], par, dt)) continue;
+            const double dtd = dt / I.dy;
+            double m = y_tile<S>(I.T, I.U, t.cg, t.j0, t.j1, dtd, 0.5 * dtd, I.bc, 1, 1, lane);
+            m = wave_max(m);
+            if (lane == 0 && m > 0.0)
+                atomicMax(&a.clk[t.inst].smax[par ^ 1], (unsigned long long)__double_as_longlong(m));
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            __threadfence();
+            if (atomicAdd(&a.ctrl->done[par], 1u) == gridDim.x - 1)
+                __hip_atomic_store(&a.ctrl->step, step + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+}
+
+// Initial max signal speed of every instance into slot 0.
+__global__ __launch_bounds__(256) void k_batch_speed(const BatchInst* inst, BatchClock* clk, int ninst) {
+    const int k = blockIdx.y;
+    if (k >= ninst) return;
+    const View v = inst[k].U;
+    const int64_t n = (int64_t)v.nx * v.ny;
+    double m = 0.0;
+    for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (int64_t)gridDim.x * blockDim.x) {
+        const int j = (int)(c / v.nx);
+        const int i = (int)(c - (int64_t)j * v.nx);
+        m = smax(m, signal_speed(ld(v, i, j)));
+    }
+    m = wave_max(m);
+    if ((threadIdx.x & 63) == 0 && m > 0.0)
+        atomicMax(&clk[k].smax[0], (unsigned long long)__double_as_longlong(m));
+}
+
+}  // namespace mh
+
+using namespace mh;
+
+struct mh_batch_s {
+    int n = 0;
+    int solver = 1;
+    double cfl = 0.8;
+    double* arena = nullptr;
+    std::vector<mh_grid_s> grids;       // non-owning views of the instances
+    std::vector<BatchInst> hinst;
+    BatchInst* dinst = nullptr;
+    BatchClock* dclk = nullptr;
+    Ctrl* dctrl = nullptr;
+    XTile* dxt = nullptr;
+    YTile* dyt = nullptr;
+    int* dxs = nullptr;
+    int* dys = nullptr;
+    int nblocks = 0;
+    int nxt = 0, nyt = 0;
+    // schedule
+    int xseg = 0, ychunk = 256, two_launch = 0, blocks_per_cu = 0;
+    hipGraphExec_t graph = nullptr;
+    hipGraph_t graph_def = nullptr;
+    int graph_steps = 0;
+};
+
+namespace {
+
+int64_t grid_rp(int nx) { return round_up((int64_t)kLead + nx + kNg, 16); }
+int64_t grid_ps(int nx, int ny) { return round_up(grid_rp(nx) * (ny + 2 * kNg), 256); }
+
+void free_dev(void* p) {
+    if (p) (void)hipFree(p);
+}
+
+void drop_graph(mh_batch b) {
+    if (b->graph) (void)hipGraphExecDestroy(b->graph);
+    if (b->graph_def) (void)hipGraphDestroy(b->graph_def);
+    b->graph = nullptr;
+    b->graph_def = nullptr;
+    b->graph_steps = 0;
+}
+
+const void* batch_kernel(int solver) {
+    switch (solver) {
+        case kHLL: return (const void*)k_batch_step<kHLL>;
+        case kHLLC: return (const void*)k_batch_step<kHLLC>;
+        case kExact: return (const void*)k_batch_step<kExact>;
+        default: return (const void*)k_batch_step<kForce>;
+    }
+}
+
+// Contiguous ranges of equal cost over `cost` for nw wavefronts.
+std::vector<int> partition(const std::vector<double>& cost, int nw) {
+    std::vector<int> start(nw + 1, (int)cost.size());
+    double total = 0.0;
+    for (double c : cost) total += c;
+    double acc = 0.0;
+    int q = 0;
+    for (int w = 0; w < nw; ++w) {
+        start[w] = q;
+        const double target = total * (double)(w + 1) / (double)nw;
+        while (q < (int)cost.size() && acc + 0.5 * cost[q] <= target) acc += cost[q++];
+    }
+    start[nw] = (int)cost.size();
+    return start;
+}
+
+int build_schedule(mh_batch b) {
+    drop_graph(b);
+    int dev = 0;
+    int rc = check_hip(hipGetDevice(&dev), "hipGetDevice");
+    if (rc) return rc;
+    hipDeviceProp_t prop;
+    rc = check_hip(hipGetDeviceProperties(&prop, dev), "hipGetDeviceProperties");
+    if (rc) return rc;
+    int occ = 0;
+    rc = check_hip(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, batch_kernel(b->solver), 256, 0),
+                   "hipOccupancyMaxActiveBlocksPerMultiprocessor");
+    if (rc) return rc;
+    occ = std::max(occ, 1);
+    if (b->blocks_per_cu > 0) occ = std::min(occ, b->blocks_per_cu);
+    b->nblocks = prop.multiProcessorCount * occ;
+    const int nw = b->nblocks * 4;
+    std::vector<XTile> xt;
+    std::vector<YTile> yt;
+    std::vector<double> xc, yc;
+    for (int k = 0; k < b->n; ++k) {
+        const int nx = b->hinst[k].U.nx, ny = b->hinst[k].U.ny;
+        const int seg = b->xseg > 0 ? b->xseg : nx;
+        for (int j = 0; j < ny; ++j)
+            for (int i0 = 0; i0 < nx; i0 += seg) {
+                const int i1 = std::min(nx, i0 + seg);
+                xt.push_back({k, j, i0, i1});
+                xc.push_back(64.0 * ((i1 - i0 + 4 + 63) / 64));
+            }
+        const int ch = b->ychunk > 0 ? b->ychunk : ny;
+        for (int cg = 0; cg * 64 < nx; ++cg)
+            for (int j0 = 0; j0 < ny; j0 += ch) {
+                const int j1 = std::min(ny, j0 + ch);
+                yt.push_back({k, cg, j0, j1});
+                yc.push_back(64.0 * (j1 - j0 + 4));
+            }
+    }
+    const std::vector<int> xs = partition(xc, nw), ys = partition(yc, nw);
+    free_dev(b->dxt); free_dev(b->dyt); free_dev(b->dxs); free_dev(b->dys);
+    b->dxt = nullptr; b->dyt = nullptr; b->dxs = nullptr; b->dys = nullptr;
+    rc = check_hip(hipMalloc((void**)&b->dxt, sizeof(XTile) * xt.size()), "hipMalloc(tiles)");
+    if (!rc) rc = check_hip(hipMalloc((void**)&b->dyt, sizeof(YTile) * yt.size()), "hipMalloc(tiles)");
+    if (!rc) rc = check_hip(hipMalloc((void**)&b->dxs, sizeof(int) * xs.size()), "hipMalloc(tiles)");
+    if (!rc) rc = check_hip(hipMalloc((void**)&b->dys, sizeof(int) * ys.size()), "hipMalloc(tiles)");
+    if (!rc) rc = check_hip(hipMemcpy(b->dxt, xt.data(), sizeof(XTile) * xt.size(), hipMemcpyHostToDevice), "hipMemcpy");
+    if (!rc) rc = check_hip(hipMemcpy(b->dyt, yt.data(), sizeof(YTile) * yt.size(), hipMemcpyHostToDevice), "hipMemcpy");
+    if (!rc) rc = check_hip(hipMemcpy(b->dxs, xs.data(), sizeof(int) * xs.size(), hipMemcpyHostToDevice), "hipMemcpy");
+    if (!rc) rc = check_hip(hipMemcpy(b->dys, ys.data(), sizeof(int) * ys.size(), hipMemcpyHostToDevice), "hipMemcpy");
+    b->nxt = (int)xt.size();
+    b->nyt = (int)yt.size();
+    return rc;
+}
+
+int launch_step(mh_batch b, hipStream_t st) {
+    BatchArgs a;
+    a.inst = b->dinst;
+    a.clk = b->dclk;
+    a.ninst = b->n;
+    a.xt = b->dxt;
+    a.xstart = b->dxs;
+    a.yt = b->dyt;
+    a.ystart = b->dys;
+    a.ctrl = b->dctrl;
+    const dim3 g(b->nblocks), blk(256);
+    const int np = b->two_launch ? 2 : 1;
+    for (int k = 0; k < np; ++k) {
+        a.phases = b->two_launch ? (k == 0 ? 1 : 2) : 3;
+        switch (b->solver) {
+            case kHLL: hipLaunchKernelGGL(k_batch_step<kHLL>, g, blk, 0, st, a); break;
+            case kHLLC: hipLaunchKernelGGL(k_batch_step<kHLLC>, g, blk, 0, st, a); break;
+            case kExact: hipLaunchKernelGGL(k_batch_step<kExact>, g, blk, 0, st, a); break;
+            default: hipLaunchKernelGGL(k_batch_step<kForce>, g, blk, 0, st, a); break;
+        }
+    }
+    return check_hip(hipGetLastError(), "batch step launch");
+}
+
+}  // namespace
+
+extern "C" {
+
+int mh_batch_create(int n, const mh_grid_desc* descs, int solver, double cfl, const mh_bc* bcs, mh_batch* out) {
+    if (n < 1 || !descs || !out) return fail(MH_ERR_INVALID, "mh_batch_create: bad argument");
+    if (solver < 0 || solver > 3) return fail(MH_ERR_INVALID, "mh_batch_create: unknown solver");
+    if (!(cfl > 0.0)) return fail(MH_ERR_INVALID, "mh_batch_create: cfl must be positive");
+    for (int k = 0; k < n; ++k)
+        if (descs[k].nx < 1 || descs[k].ny < 1 || !(descs[k].x_max > descs[k].x_min) ||
+            !(descs[k].y_max > descs[k].y_min))
+            return fail(MH_ERR_INVALID, "mh_batch_create: bad instance descriptor");
+    if (bcs)
+        for (int k = 0; k < n; ++k)
+            for (int e = 0; e < 4; ++e)
+                if (bcs[k].kind[e] != MH_BC_TRANSMISSIVE && bcs[k].kind[e] != MH_BC_INFLOW)
+                    return fail(MH_ERR_INVALID, "mh_batch_create: unknown boundary kind");
+    mh_batch b = new (std::nothrow) mh_batch_s();
+    if (!b) return fail(MH_ERR_OOM, "host allocation failed");
+    b->n = n;
+    b->solver = solver;
+    b->cfl = cfl;
+    std::vector<int64_t> off(n);
+    int64_t total = 0;
+    for (int k = 0; k < n; ++k) {
+        off[k] = total;
+        total += 2 * 4 * grid_ps(descs[k].nx, descs[k].ny);
+    }
+    int rc = check_hip(hipMalloc((void**)&b->arena, sizeof(double) * (size_t)total), "hipMalloc(batch arena)");
+    if (rc) { delete b; return rc; }
+    b->grids.resize(n);
+    b->hinst.resize(n);
+    std::vector<BatchClock> clk(n);
+    for (int k = 0; k < n; ++k) {
+        mh_grid_s& g = b->grids[k];
+        g.d = descs[k];
+        g.dx = (descs[k].x_max - descs[k].x_min) / (double)descs[k].nx;
+        g.dy = (descs[k].y_max - descs[k].y_min) / (double)descs[k].ny;
+        g.rp = grid_rp(descs[k].nx);
+        g.ps = grid_ps(descs[k].nx, descs[k].ny);
+        g.base = b->arena + off[k];
+        g.aux = nullptr;
+        g.owns = 0;
+        (void)hipGetDevice(&g.device);
+        mh_grid_s t = g;
+        t.base = g.base + 4 * g.ps;
+        BatchInst& I = b->hinst[k];
+        I.U = g.view();
+        I.T = t.view();
+        I.dx = g.dx;
+        I.dy = g.dy;
+        I.cfl = cfl;
+        I.h = std::min(g.dx, g.dy);
+        I.bc = to_bc(bcs ? &bcs[k] : nullptr);
+        std::memset(&clk[k], 0, sizeof(BatchClock));
+    }
+    rc = check_hip(hipMalloc((void**)&b->dinst, sizeof(BatchInst) * n), "hipMalloc(batch)");
+    if (!rc) rc = check_hip(hipMalloc((void**)&b->dclk, sizeof(BatchClock) * n), "hipMalloc(batch)");
+    if (!rc) rc = check_hip(hipMalloc((void**)&b->dctrl, sizeof(Ctrl)), "hipMalloc(batch)");
+    if (!rc) rc = check_hip(hipMemcpy(b->dinst, b->hinst.data(), sizeof(BatchInst) * n, hipMemcpyHostToDevice), "hipMemcpy");
+    if (!rc) rc = check_hip(hipMemcpy(b->dclk, clk.data(), sizeof(BatchClock) * n, hipMemcpyHostToDevice), "hipMemcpy");
+    Ctrl c0{};
+    if (!rc) rc = check_hip(hipMemcpy(b->dctrl, &c0, sizeof(Ctrl), hipMemcpyHostToDevice), "hipMemcpy");
+    if (!rc) rc = build_schedule(b);
+    if (rc) { mh_batch_destroy(b); return rc; }
+    *out = b;
+    return MH_OK;
+}
+
+int mh_batch_destroy(mh_batch b) {
+    if (!b) return MH_OK;
+    drop_graph(b);
+    free_dev(b->arena); free_dev(b->dinst); free_dev(b->dclk); free_dev(b->dctrl);
+    free_dev(b->dxt); free_dev(b->dyt); free_dev(b->dxs); free_dev(b->dys);
+    delete b;
+    return MH_OK;
+}
+
+int mh_batch_grid(mh_batch b, int k, mh_grid* g) {
+    if (!b || k < 0 || k >= b->n || !g) return fail(MH_ERR_INVALID, "mh_batch_grid: bad argument");
+    *g = &b->grids[k];
+    return MH_OK;
+}
+
+int mh_batch_prepare(mh_batch b, mh_stream stream) {
+    if (!b) return fail(MH_ERR_INVALID, "mh_batch_prepare: null batch");
+    hipStream_t st = (hipStream_t)stream;
+    int rc = MH_OK;
+    for (int k = 0; k < b->n && !rc; ++k) rc = launch_fill(&b->grids[k], b->hinst[k].bc, st);
+    std::vector<BatchClock> clk(b->n);
+    for (auto& c : clk) std::memset(&c, 0, sizeof(c));
+    Ctrl c0{};
+    if (!rc) rc = check_hip(hipMemcpyAsync(b->dclk, clk.data(), sizeof(BatchClock) * b->n, hipMemcpyHostToDevice, st), "hipMemcpyAsync");
+    if (!rc) rc = check_hip(hipMemcpyAsync(b->dctrl, &c0, sizeof(Ctrl), hipMemcpyHostToDevice, st), "hipMemcpyAsync");
+    int maxcells = 0;
+    for (auto& I : b->hinst) maxcells = std::max(maxcells, I.U.nx * I.U.ny);
+    if (!rc) {
+        hipLaunchKernelGGL(k_batch_speed, dim3(std::max(1, std::min(64, (maxcells + 255) / 256)), b->n), dim3(256), 0, st,
+                           b->dinst, b->dclk, b->n);
+        rc = check_hip(hipGetLastError(), "batch speed launch");
+    }
+    if (!rc) rc = check_hip(hipStreamSynchronize(st), "hipStreamSynchronize");
+    return rc;
+}
+
+int mh_batch_set_schedule(mh_batch b, int xseg, int ychunk, int two_launch, int blocks_per_cu) {
+    if (!b || xseg < 0 || ychunk < 0 || blocks_per_cu < 0) return fail(MH_ERR_INVALID, "mh_batch_set_schedule: bad argument");
+    b->xseg = xseg;
+    b->ychunk = ychunk;
+    b->two_launch = two_launch ? 1 : 0;
+    b->blocks_per_cu = blocks_per_cu;
+    return build_schedule(b);
+}
+
+int mh_batch_step(mh_batch b, int nsteps, mh_stream stream) {
+    if (!b || nsteps < 0) return fail(MH_ERR_INVALID, "mh_batch_step: bad argument");
+    hipStream_t st = (hipStream_t)stream;
+    int k = 0;
+    if (b->graph && b->graph_steps > 0)
+        for (; k + b->graph_steps <= nsteps; k += b->graph_steps) {
+            const int rc = check_hip(hipGraphLaunch(b->graph, st), "hipGraphLaunch");
+            if (rc) return rc;
+        }
+    for (; k < nsteps; ++k) {
+        const int rc = launch_step(b, st);
+        if (rc) return rc;
+    }
+    return MH_OK;
+}
+
+int mh_batch_capture(mh_batch b, int steps_per_graph, mh_stream stream) {
+    if (!b || steps_per_graph < 1) return fail(MH_ERR_INVALID, "mh_batch_capture: bad argument");
+    drop_graph(b);
+    hipStream_t st = (hipStream_t)stream;
+    hipStream_t cap = nullptr;
+    int rc = check_hip(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking), "hipStreamCreate");
+    if (rc) return rc;
+    rc = check_hip(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+    for (int k = 0; k < steps_per_graph && !rc; ++k) rc = launch_step(b, cap);
+    hipGraph_t g = nullptr;
+    const int rc2 = check_hip(hipStreamEndCapture(cap, &g), "hipStreamEndCapture");
+    if (!rc) rc = rc2;
+    if (!rc) rc = check_hip(hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0), "hipGraphInstantiate");
+    if (!rc) {
+        b->graph_def = g;
+        b->graph_steps = steps_per_graph;
+    } else if (g) {
+        (void)hipGraphDestroy(g);
+    }
+    (void)hipStreamDestroy(cap);
+    (void)st;
+    return rc;
+}
+
+int mh_batch_clock(mh_batch b, double* t, double* dt, int* steps, int* status, mh_stream stream) {
+    if (!b) return fail(MH_ERR_INVALID, "mh_batch_clock: null batch");
+    hipStream_t st = (hipStream_t)stream;
+    std::vector<BatchClock> clk(b->n);
+    Ctrl c;
+    int rc = check_hip(hipMemcpyAsync(clk.data(), b->dclk, sizeof(BatchClock) * b->n, hipMemcpyDeviceToHost, st), "hipMemcpyAsync");
+    if (!rc) rc = check_hip(hipMemcpyAsync(&c, b->dctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, st), "hipMemcpyAsync");
+    if (!rc) rc = check_hip(hipStreamSynchronize(st), "hipStreamSynchronize");
+    if (rc) return rc;
+    int bad = 0;
+    for (int k = 0; k < b->n; ++k) {
+        if (t) t[k] = clk[k].t;
+        if (dt) dt[k] = clk[k].dt;
+        if (steps) steps[k] = clk[k].step;
+        if (status) status[k] = clk[k].status;
+        bad |= clk[k].status;
+    }
+    if (c.err) return fail(MH_ERR_LAUNCH, "batch: grid barrier timed out (workgroups not co-resident)");
+    if (bad) return fail(MH_ERR_WAVESPEED, "batch: an instance met a non-positive max wave speed");
+    return MH_OK;
+}
+
+int mh_batch_info(mh_batch b, int* nblocks, int* nxtiles, int* nytiles) {
+    if (!b) return fail(MH_ERR_INVALID, "mh_batch_info: null batch");
+    if (nblocks) *nblocks = b->nblocks;
+    if (nxtiles) *nxtiles = b->nxt;
+    if (nytiles) *nytiles = b->nyt;
+    return MH_OK;
+}
+
+}  // extern "C"

@@ -37,8 +37,9 @@ struct mh_grid_s {
     double dx, dy;
     int64_t rp, ps;      // row pitch, plane stride (doubles)
     double* base;        // allocation start
-    unsigned long long* aux;  // 8 device words of per-grid scratch (max-speed slot)
+    unsigned long long* aux;  // 8 device words of per-grid scratch (max-speed slot), or null
     int device;
+    int owns;            // 1: base/aux were allocated for this grid (0: a view into a batch arena)
     mh::View view() const {
         mh::View v;
         v.o = base + (int64_t)mh::kNg * rp + mh::kLead;

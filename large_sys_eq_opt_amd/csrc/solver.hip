@@ -33,6 +33,7 @@
 
 #include "euler_device.h"
 #include "grid.h"
+#include "sweep_tiles.h"
 
 namespace mh {
 
@@ -52,26 +53,8 @@ int check_hip(hipError_t e, const char* what) {
 }
 
 // ---------------------------------------------------------------------------
-// device helpers
+// device clock and sweep kernels
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ Cons ld(const View& v, int i, int j) {
-    const double* q = v.o + (int64_t)j * v.rp + i;
-    return {q[0], q[v.ps], q[2 * v.ps], q[3 * v.ps]};
-}
-__device__ __forceinline__ void st(const View& v, int i, int j, const Cons& U) {
-    double* q = v.o + (int64_t)j * v.rp + i;
-    q[0] = U.r;
-    q[v.ps] = U.mu;
-    q[2 * v.ps] = U.mv;
-    q[3 * v.ps] = U.e;
-}
-
-// Boundary spec in kernel-argument form.
-struct Bc {
-    int kind[4];
-    Cons s[4];
-};
-
 // Device clock of the device-resident time loop.
 struct Clock {
     double t, dt, t_end, cfl, h;
@@ -95,22 +78,10 @@ __device__ __forceinline__ bool dt_of(const DtSrc& s, double& dt) {
     return true;
 }
 
-__device__ __forceinline__ double wave_max(double m) {
-    for (int off = 32; off >= 1; off >>= 1) m = smax(m, __shfl_xor(m, off, 64));
-    return m;
-}
-
-// Contribution of one cell's signal speed to a running max with the
-// semantics of std::max(running, s) (NaN speeds never enter).
-__device__ __forceinline__ double run_max(double m, double s) { return smax(m, s); }
-
 __device__ __forceinline__ void atomic_max_pos(unsigned long long* slot, double m) {
     if (m > 0.0) atomicMax(slot, (unsigned long long)__double_as_longlong(m));
 }
 
-// ---------------------------------------------------------------------------
-// y sweep
-// ---------------------------------------------------------------------------
 struct SweepArgs {
     DtSrc dts;
     double d;          // dx or dy
@@ -130,84 +101,20 @@ __global__ __launch_bounds__(64) void k_sweep_y(View in, View out, SweepArgs a) 
     if (!dt_of(a.dts, dt)) return;
     const double dtd = dt / a.d;
     const double half = 0.5 * dtd;
-    const int lane = threadIdx.x;
     const int cg = blockIdx.x % a.ngroups;
     const int ch = blockIdx.x / a.ngroups;
-    const int i = cg * 64 + lane;
-    const bool active = i < in.nx;
-    const int ic = active ? i : in.nx - 1;
     const int j0 = ch * a.seg;
     const int j1 = min(j0 + a.seg, in.ny);
-    int it = 0;
-
-    // p = j0-2, j0-1: load and convert
-    Cons U2 = ld(in, ic, j0 - 2), U1 = ld(in, ic, j0 - 1);
-    Prim W2 = c2p(U2), W1 = c2p(U1);
-    Cons Un = ld(in, ic, j0);
-    // p = j0: reconstruct q = j0-1
-    Edge L, R, Rp;
-    {
-        const Cons U0 = Un;
-        Un = ld(in, ic, j0 + 1);
-        const Prim W0 = c2p(U0);
-        reconstruct<kY>(W2, W1, W0, half, L, Rp);
-        U2 = U1; U1 = U0; W2 = W1; W1 = W0;
-    }
-    // p = j0+1: reconstruct q = j0, flux at face j0-1/2
-    Cons Fp;
-    {
-        const Cons U0 = Un;
-        Un = ld(in, ic, j0 + 2);
-        const Prim W0 = c2p(U0);
-        reconstruct<kY>(W2, W1, W0, half, L, R);
-        Fp = riemann<S, kY>(Rp.U, Rp.V, L.U, L.V, it);
-        Rp = R;
-        U2 = U1; U1 = U0; W2 = W1; W1 = W0;
-    }
-    double m = 0.0;
-    // p = j0+2 .. j1+1: reconstruct q = p-1, face q-1/2, update row r = p-2
-    for (int p = j0 + 2; p <= j1 + 1; ++p) {
-        const Cons U0 = Un;
-        Un = ld(in, ic, min(p + 1, j1 + 1));
-        const Prim W0 = c2p(U0);
-        reconstruct<kY>(W2, W1, W0, half, L, R);
-        const Cons F = riemann<S, kY>(Rp.U, Rp.V, L.U, L.V, it);
-        const int r = p - 2;
-        const Cons Unew = update(U2, dtd, Fp, F);
-        if (active) {
-            st(out, i, r, Unew);
-            if (a.speed || a.clock_speed) m = run_max(m, signal_speed(Unew));
-            if (a.ghosts) {
-                if (i == 0) {
-                    const Cons G = a.bc.kind[MH_EDGE_LEFT] == MH_BC_INFLOW ? a.bc.s[MH_EDGE_LEFT] : Unew;
-                    st(out, -1, r, G);
-                    st(out, -2, r, G);
-                }
-                if (i == in.nx - 1) {
-                    const Cons G = a.bc.kind[MH_EDGE_RIGHT] == MH_BC_INFLOW ? a.bc.s[MH_EDGE_RIGHT] : Unew;
-                    st(out, in.nx, r, G);
-                    st(out, in.nx + 1, r, G);
-                }
-            }
-        }
-        Fp = F;
-        Rp = R;
-        U2 = U1; U1 = U0; W2 = W1; W1 = W0;
-    }
-    if (a.speed || a.clock_speed) {
+    const int want = (a.speed || a.clock_speed) ? 1 : 0;
+    double m = y_tile<S>(in, out, cg, j0, j1, dtd, half, a.bc, a.ghosts, want, threadIdx.x);
+    if (want) {
         m = wave_max(m);
-        if (lane == 0) {
+        if (threadIdx.x == 0) {
             unsigned long long* slot = a.clock_speed ? &a.dts.clock->smax[a.dts.clock->slot ^ 1] : a.speed;
             atomic_max_pos(slot, m);
         }
     }
 }
-
-// ---------------------------------------------------------------------------
-// x sweep
-// ---------------------------------------------------------------------------
-// LDS exchange slots: 2 carry slots + 64 lanes.
-constexpr int kSlots = 66;
 
 template <int S>
 __global__ __launch_bounds__(64) void k_sweep_x(View in, View out, SweepArgs a) {
@@ -215,70 +122,12 @@ __global__ __launch_bounds__(64) void k_sweep_x(View in, View out, SweepArgs a) 
     if (!dt_of(a.dts, dt)) return;
     const double dtd = dt / a.d;
     const double half = 0.5 * dtd;
-    const int lane = threadIdx.x;
     const int j = blockIdx.x / a.ngroups;
     const int sg = blockIdx.x - j * a.ngroups;
     const int i0 = sg * a.seg;
     const int i1 = min(i0 + a.seg, in.nx);
-    __shared__ double sW[4][kSlots];
-    __shared__ double sR[8][kSlots];
-    __shared__ double sF[4][kSlots];
-    int it = 0;
-    const int niter = (i1 - i0 + 4 + 63) / 64;
-    Cons Un = ld(in, min(i0 - 2 + lane, in.nx + 1), j);
-    for (int k = 0; k < niter; ++k) {
-        const int p = i0 - 2 + 64 * k + lane;   // cell loaded by this lane
-        const Cons U0 = Un;
-        Un = ld(in, min(p + 64, in.nx + 1), j);
-        const Prim W0 = c2p(U0);
-        // W_{p-1}, W_{p-2}
-        sW[0][2 + lane] = W0.r; sW[1][2 + lane] = W0.u; sW[2][2 + lane] = W0.v; sW[3][2 + lane] = W0.p;
-        __syncthreads();
-        const Prim W1 = {sW[0][1 + lane], sW[1][1 + lane], sW[2][1 + lane], sW[3][1 + lane]};
-        const Prim W2 = {sW[0][lane], sW[1][lane], sW[2][lane], sW[3][lane]};
-        __syncthreads();
-        if (lane >= 62) {
-            sW[0][lane - 62] = W0.r; sW[1][lane - 62] = W0.u; sW[2][lane - 62] = W0.v; sW[3][lane - 62] = W0.p;
-        }
-        // reconstruct q = p-1
-        Edge L, R;
-        reconstruct<kX>(W2, W1, W0, half, L, R);
-        // right edge state of cell q-1
-        sR[0][2 + lane] = R.U.r; sR[1][2 + lane] = R.U.mu; sR[2][2 + lane] = R.U.mv; sR[3][2 + lane] = R.U.e;
-        sR[4][2 + lane] = R.V.r; sR[5][2 + lane] = R.V.u; sR[6][2 + lane] = R.V.v; sR[7][2 + lane] = R.V.p;
-        __syncthreads();
-        const Cons RU = {sR[0][1 + lane], sR[1][1 + lane], sR[2][1 + lane], sR[3][1 + lane]};
-        const Prim RV = {sR[4][1 + lane], sR[5][1 + lane], sR[6][1 + lane], sR[7][1 + lane]};
-        __syncthreads();
-        if (lane == 63) {
-            sR[0][1] = R.U.r; sR[1][1] = R.U.mu; sR[2][1] = R.U.mv; sR[3][1] = R.U.e;
-            sR[4][1] = R.V.r; sR[5][1] = R.V.u; sR[6][1] = R.V.v; sR[7][1] = R.V.p;
-        }
-        // face q-1/2 = r+1/2 with r = p-2
-        const Cons F = riemann<S, kX>(RU, RV, L.U, L.V, it);
-        sF[0][2 + lane] = F.r; sF[1][2 + lane] = F.mu; sF[2][2 + lane] = F.mv; sF[3][2 + lane] = F.e;
-        __syncthreads();
-        const Cons Fm = {sF[0][1 + lane], sF[1][1 + lane], sF[2][1 + lane], sF[3][1 + lane]};
-        __syncthreads();
-        if (lane == 63) { sF[0][1] = F.r; sF[1][1] = F.mu; sF[2][1] = F.mv; sF[3][1] = F.e; }
-        const int r = p - 2;
-        if (r >= i0 && r < i1) {
-            const Cons Unew = update(ld(in, r, j), dtd, Fm, F);
-            st(out, r, j, Unew);
-            if (a.ghosts) {
-                if (j == 0) {
-                    const Cons G = a.bc.kind[MH_EDGE_BOTTOM] == MH_BC_INFLOW ? a.bc.s[MH_EDGE_BOTTOM] : Unew;
-                    st(out, r, -1, G);
-                    st(out, r, -2, G);
-                }
-                if (j == in.ny - 1) {
-                    const Cons G = a.bc.kind[MH_EDGE_TOP] == MH_BC_INFLOW ? a.bc.s[MH_EDGE_TOP] : Unew;
-                    st(out, r, in.ny, G);
-                    st(out, r, in.ny + 1, G);
-                }
-            }
-        }
-    }
+    __shared__ XLds X;
+    x_tile<S>(in, out, j, i0, i1, dtd, half, a.bc, a.ghosts, X, threadIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -310,7 +159,7 @@ __global__ __launch_bounds__(256) void k_max_speed(View v, unsigned long long* s
     for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (int64_t)gridDim.x * blockDim.x) {
         const int j = (int)(c / v.nx);
         const int i = (int)(c - (int64_t)j * v.nx);
-        m = run_max(m, signal_speed(ld(v, i, j)));
+        m = smax(m, signal_speed(ld(v, i, j)));
     }
     m = wave_max(m);
     if ((threadIdx.x & 63) == 0) atomic_max_pos(slot, m);
@@ -562,6 +411,7 @@ int mh_grid_create(const mh_grid_desc* desc, mh_grid* out) {
     mh_grid g = new (std::nothrow) mh_grid_s();
     if (!g) return fail(MH_ERR_OOM, "host allocation failed");
     g->d = *desc;
+    g->owns = 1;
     g->dx = (desc->x_max - desc->x_min) / (double)desc->nx;   // head/gpu/grid_gpu.cuh:40-41
     g->dy = (desc->y_max - desc->y_min) / (double)desc->ny;
     g->rp = round_up((int64_t)kLead + desc->nx + kNg, 16);
@@ -584,6 +434,7 @@ int mh_grid_create(const mh_grid_desc* desc, mh_grid* out) {
 
 int mh_grid_destroy(mh_grid g) {
     if (!g) return MH_OK;
+    if (!g->owns) return fail(MH_ERR_INVALID, "mh_grid_destroy: grid belongs to a batch");
     hipError_t e = hipFree(g->base);
     const hipError_t e2 = hipFree(g->aux);
     if (e == hipSuccess) e = e2;
@@ -694,6 +545,7 @@ int mh_fill_ghosts(mh_grid g, const mh_bc* bc, mh_stream stream) {
 
 int mh_compute_dt(mh_grid g, double cfl, double* dt, double* max_speed, mh_stream stream) {
     if (!g) return fail(MH_ERR_INVALID, "mh_compute_dt: null grid");
+    if (!g->aux) return fail(MH_ERR_INVALID, "mh_compute_dt: grid of a batch (use mh_batch_clock)");
     hipStream_t st = (hipStream_t)stream;
     unsigned long long* slot = g->aux;
     hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, slot, 1);

@@ -1,0 +1,181 @@
+"""Batched ensembles of independent blast waves and the schedule autotuner.
+
+Ensemble specification (seed 0, 512 instances by default): per instance, in
+this order, three draws u1, u2, u3 from SplitMix64 mapped to [0, 1) by the top
+53 bits, then
+    side  = round(exp(ln 64 + u1 * (ln 1024 - ln 64)))   (log-uniform in [64, 1024])
+    r0    = 0.1 + 0.4 * u2                                (uniform in [0.1, 0.5])
+    p_in  = exp(u3 * ln 100)                              (log-uniform in [1, 100])
+Each instance is a side x side grid on [0,2]^2 with the blast centred at
+(1,1): rho = 1, p = p_in for r <= r0, rho = 0.125, p = 0.1 outside,
+transmissive boundaries, CFL 0.8.
+
+``Batch`` drives the persistent batch kernel (csrc/batch.hip) through the C
+ABI; ``autotune`` times candidate schedules (tile shapes, one fused launch or
+two launches per step, workgroups per CU) on the batch itself and keeps the
+fastest.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import json
+import math
+import os
+import time
+
+import numpy as np
+
+from . import _lib
+from ._lib import GridDesc, call
+from .solver import Grid2DGPU, _stream
+
+MASK64 = (1 << 64) - 1
+
+
+def splitmix64(seed: int):
+    x = seed & MASK64
+    while True:
+        x = (x + 0x9E3779B97F4A7C15) & MASK64
+        z = x
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & MASK64
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & MASK64
+        yield z ^ (z >> 31)
+
+
+def ensemble_spec(n: int = 512, seed: int = 0):
+    g = splitmix64(seed)
+    u = lambda: (next(g) >> 11) * (1.0 / 9007199254740992.0)  # noqa: E731
+    out = []
+    for _ in range(n):
+        u1, u2, u3 = u(), u(), u()
+        side = int(round(math.exp(math.log(64.0) + u1 * (math.log(1024.0) - math.log(64.0)))))
+        out.append({"side": side, "r0": 0.1 + 0.4 * u2, "p_in": math.exp(u3 * math.log(100.0))})
+    return out
+
+
+def blast_params(inst):
+    """mh_init_state kind-2 parameters (x0, y0, r0, rho_in, p_in, rho_out, p_out)."""
+    return np.array([1.0, 1.0, inst["r0"], 1.0, inst["p_in"], 0.125, 0.1])
+
+
+class Batch:
+    def __init__(self, spec, solver=1, cfl=0.8, domain=(0.0, 2.0, 0.0, 2.0)):
+        self.spec = list(spec)
+        self.n = len(self.spec)
+        self.solver = int(solver)
+        descs = (GridDesc * self.n)()
+        for k, s in enumerate(self.spec):
+            descs[k] = GridDesc(s["side"], s["side"], *map(float, domain))
+        self._h = C.c_void_p()
+        call("mh_batch_create", self.n, descs, self.solver, float(cfl), None, C.byref(self._h))
+        self.cells = sum(s["side"] * s["side"] for s in self.spec)
+        self.schedule = (0, 256, 0, 0)
+
+    def grid(self, k: int) -> Grid2DGPU:
+        """A Grid2DGPU-like view of instance k (non-owning)."""
+        h = C.c_void_p()
+        call("mh_batch_grid", self._h, int(k), C.byref(h))
+        g = Grid2DGPU.__new__(Grid2DGPU)
+        g._h = h
+        s = self.spec[k]
+        g.nx = g.ny = s["side"]
+        g.ng = 2
+        g.x_min, g.x_max, g.y_min, g.y_max = 0.0, 2.0, 0.0, 2.0
+        g.close = lambda: None   # owned by the batch
+        return g
+
+    def initialise(self):
+        for k, s in enumerate(self.spec):
+            self.grid(k).init_state(2, blast_params(s))
+        call("mh_batch_prepare", self._h, None)
+
+    def set_schedule(self, xseg=0, ychunk=256, two_launch=0, blocks_per_cu=0):
+        call("mh_batch_set_schedule", self._h, int(xseg), int(ychunk), int(two_launch), int(blocks_per_cu))
+        self.schedule = (int(xseg), int(ychunk), int(two_launch), int(blocks_per_cu))
+
+    def capture(self, steps_per_graph: int, stream=None):
+        call("mh_batch_capture", self._h, int(steps_per_graph), _stream(stream))
+
+    def step(self, nsteps: int, stream=None):
+        call("mh_batch_step", self._h, int(nsteps), _stream(stream))
+
+    def clock(self, stream=None):
+        t, dt = np.zeros(self.n), np.zeros(self.n)
+        st, ss = np.zeros(self.n, np.int32), np.zeros(self.n, np.int32)
+        call("mh_batch_clock", self._h, t.ctypes.data_as(C.c_void_p), dt.ctypes.data_as(C.c_void_p),
+             st.ctypes.data_as(C.c_void_p), ss.ctypes.data_as(C.c_void_p), _stream(stream))
+        return t, dt, st, ss
+
+    def info(self):
+        a, b, c = C.c_int(), C.c_int(), C.c_int()
+        call("mh_batch_info", self._h, C.byref(a), C.byref(b), C.byref(c))
+        return {"blocks": a.value, "x_tiles": b.value, "y_tiles": c.value}
+
+    def download(self, k: int):
+        return self.grid(k).download_to_aos()
+
+    def close(self):
+        if self._h:
+            _lib.load().mh_batch_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+DEFAULT_CANDIDATES = [
+    # (xseg, ychunk, two_launch, blocks_per_cu)
+    (0, 256, 0, 0), (0, 128, 0, 0), (0, 512, 0, 0), (0, 0, 0, 0),
+    (0, 256, 1, 0), (512, 256, 0, 0), (0, 256, 0, 2), (0, 128, 1, 0),
+]
+
+
+def _signature(batch: Batch) -> str:
+    sides = sorted(s["side"] for s in batch.spec)
+    return f"n{batch.n}-cells{batch.cells}-s{batch.solver}-min{sides[0]}-max{sides[-1]}"
+
+
+def autotune(batch: Batch, candidates=None, steps: int = 4, cache: str | None = None, log=None):
+    """Times each candidate schedule for `steps` steps (after one untimed
+    step) and applies the fastest; re-initialises the batch afterwards.
+    Results are cached per ensemble signature in `cache` (JSON)."""
+    sig = _signature(batch)
+    if cache and os.path.exists(cache):
+        with open(cache) as f:
+            known = json.load(f)
+        if sig in known:
+            best = tuple(known[sig]["schedule"])
+            batch.set_schedule(*best)
+            batch.initialise()
+            return {"schedule": best, "cached": True, "timings": known[sig].get("timings")}
+    timings = []
+    for cand in candidates or DEFAULT_CANDIDATES:
+        batch.set_schedule(*cand)
+        batch.initialise()
+        batch.step(1)
+        batch.clock()
+        t0 = time.perf_counter()
+        batch.step(steps)
+        batch.clock()
+        dt = (time.perf_counter() - t0) / steps
+        timings.append((dt, cand))
+        if log:
+            log(f"autotune {cand}: {dt * 1e3:.3f} ms/step")
+    timings.sort()
+    best = timings[0][1]
+    batch.set_schedule(*best)
+    batch.initialise()
+    res = {"schedule": best, "cached": False, "timings": [(c, t) for t, c in timings]}
+    if cache:
+        known = {}
+        if os.path.exists(cache):
+            with open(cache) as f:
+                known = json.load(f)
+        known[sig] = {"schedule": list(best), "timings": [[list(c), t] for t, c in timings]}
+        os.makedirs(os.path.dirname(cache) or ".", exist_ok=True)
+        with open(cache, "w") as f:
+            json.dump(known, f, indent=1)
+    return res
