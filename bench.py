@@ -69,6 +69,28 @@ def cpu_baseline(solver: int, budget_s: float = 12.0):
             "np": np.__version__}
 
 
+def copy_peak(nbytes: int = 4 << 30, reps: int = 10):
+    """Measured device-to-device copy bandwidth (read + write bytes / s)."""
+    import ctypes as C
+
+    from large_sys_eq_opt_amd._lib import call
+
+    a, b = C.c_void_p(), C.c_void_p()
+    call("mh_device_alloc", nbytes, C.byref(a))
+    call("mh_device_alloc", nbytes, C.byref(b))
+    n = nbytes // 8
+    call("mh_copy", a, b, n, None)
+    call("mh_synchronize", None)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        call("mh_copy", a, b, n, None)
+    call("mh_synchronize", None)
+    dt = (time.perf_counter() - t0) / reps
+    call("mh_device_free", a)
+    call("mh_device_free", b)
+    return 2.0 * nbytes / dt
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -77,7 +99,13 @@ def main():
     ap.add_argument("--size", type=int, default=8192)
     ap.add_argument("--solver", default="hllc")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-only", type=int, default=0,
+                    help="only run the copy kernel on buffers of this many MiB (counter calibration)")
     args = ap.parse_args()
+    if args.copy_only:
+        bw = copy_peak(args.copy_only << 20, reps=3)
+        print(json.dumps({"copy_bytes": args.copy_only << 20, "copy_GB_per_s": bw / 1e9}))
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; the multi-GPU path (y-slab decomposition "
                                    "with ghost-row exchange over xGMI) is a design, not a measured result"}))
@@ -144,6 +172,8 @@ def main():
     }
     st.close()
     U.close()
+    out["copy_peak_GB_per_s"] = copy_peak() / 1e9
+    out["roofline"]["frac_of_copy_peak"] = achieved / (out["copy_peak_GB_per_s"] * 1e9)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(int(solver))
         out["cpu_baseline"].pop("np", None)

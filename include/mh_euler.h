@@ -142,11 +142,19 @@ int mh_stepper_finish(mh_stepper s, mh_stream stream);
  * step of the clock kernel, the x sweep and the y sweep.  (No reference
  * counterpart: the reference times whole runs externally.) */
 int mh_stepper_advance_profiled(mh_stepper s, int nsteps, mh_stream stream, double* ms_total, double* ms_kernels);
-/* Launch shape of the sweeps: x-sweep row segment length (0 = whole row) and
- * y-sweep rows per wavefront chunk (0 = whole column).  Process-wide; set by
- * the autotuner.  No reference counterpart (src/gpu/solver_gpu.cu:830-831
- * hard-codes 16x16 blocks). */
-int mh_set_launch_shape(int seg_x, int seg_y);
+/* Launch shape of the sweeps (process-wide; chosen by the autotuner).  No
+ * reference counterpart: src/gpu/solver_gpu.cu:830-831 hard-codes 16x16
+ * blocks. */
+typedef struct {
+    int seg_x;    /* x tile: cells per row segment (0 = whole row) */
+    int seg_y;    /* y tile: rows per chunk (0 = whole column) */
+    int y_waves;  /* y sweep register target: 4 wavefronts/SIMD, or 1 = compiler's choice */
+    int x_layout; /* x tile LDS exchange: 0 compact (6.3 KB, selects), 1 66-slot (10.3 KB) */
+    int x_wpb;    /* wavefronts (tiles) per workgroup, x sweep: 1 or 4 */
+    int y_wpb;    /* wavefronts (tiles) per workgroup, y sweep: 1 or 4 */
+} mh_launch_shape;
+int mh_set_launch_shape(const mh_launch_shape* shape);
+int mh_get_launch_shape(mh_launch_shape* shape);
 
 /* ---- initial conditions on the device (point sampling at cell centres,
  *      ghosts included: src/init.cpp:29-38).  kind 0: reference shock_bubble
@@ -182,6 +190,14 @@ int mh_batch_capture(mh_batch b, int steps_per_graph, mh_stream stream);
 int mh_batch_clock(mh_batch b, double* t, double* dt, int* steps, int* status, mh_stream stream);
 /* Persistent grid size and tile counts of the current schedule. */
 int mh_batch_info(mh_batch b, int* nblocks, int* nxtiles, int* nytiles);
+
+/* ---- utilities (no reference counterpart) -------------------------------- */
+/* Plain streaming copy of n doubles (n even) between device buffers: the
+ * measured copy bandwidth and the calibration of HBM traffic counters. */
+int mh_copy(const double* src, double* dst, int64_t n, mh_stream stream);
+int mh_device_alloc(int64_t bytes, void** ptr);
+int mh_device_free(void* ptr);
+int mh_synchronize(mh_stream stream);
 
 /* ---- pointwise entry points over host arrays of n states (synchronous) ---- */
 /* riemann_flux (head/riemann.hpp:650-672): F = flux(UL, UR) along dir for n

@@ -28,6 +28,10 @@ class GridDesc(C.Structure):
     _fields_ = [("nx", _I), ("ny", _I), ("x_min", _D), ("x_max", _D), ("y_min", _D), ("y_max", _D)]
 
 
+class LaunchShape(C.Structure):
+    _fields_ = [("seg_x", _I), ("seg_y", _I), ("y_waves", _I), ("x_layout", _I), ("x_wpb", _I), ("y_wpb", _I)]
+
+
 class BcDesc(C.Structure):
     _fields_ = [("kind", _I * 4), ("state", (_D * 4) * 4)]
 
@@ -60,7 +64,8 @@ SIGNATURES = {
     "mh_stepper_finish": ([_P, _P], _I),
     "mh_init_state": ([_P, _I, _P, _P], _I),
     "mh_stepper_advance_profiled": ([_P, _I, _P, C.POINTER(_D), C.POINTER(_D)], _I),
-    "mh_set_launch_shape": ([_I, _I], _I),
+    "mh_set_launch_shape": ([C.POINTER(LaunchShape)], _I),
+    "mh_get_launch_shape": ([C.POINTER(LaunchShape)], _I),
     "mh_batch_create": ([_I, _P, _I, _D, _P, C.POINTER(_P)], _I),
     "mh_batch_destroy": ([_P], _I),
     "mh_batch_grid": ([_P, _I, C.POINTER(_P)], _I),
@@ -70,6 +75,10 @@ SIGNATURES = {
     "mh_batch_capture": ([_P, _I, _P], _I),
     "mh_batch_clock": ([_P, _P, _P, _P, _P, _P], _I),
     "mh_batch_info": ([_P, C.POINTER(_I), C.POINTER(_I), C.POINTER(_I)], _I),
+    "mh_copy": ([_P, _P, _I64, _P], _I),
+    "mh_device_alloc": ([_I64, C.POINTER(_P)], _I),
+    "mh_device_free": ([_P], _I),
+    "mh_synchronize": ([_P], _I),
     "mh_riemann_flux": ([_I, _P, _P, _I, _I, _P, _P], _I),
     "mh_reconstruct": ([_I, _P, _P, _P, _D, _I, _P, _P], _I),
 }
@@ -77,11 +86,13 @@ SIGNATURES = {
 _lib = None
 
 
-def load(path: str = LIB_PATH):
-    """Loads the HIP library once; raises MHError if it is not built."""
+def load(path: str | None = None):
+    """Loads the HIP library once (``MH_EULER_LIB`` names another build of
+    it); raises MHError if it is not built."""
     global _lib
     if _lib is not None:
         return _lib
+    path = path or os.environ.get("MH_EULER_LIB") or LIB_PATH
     if not os.path.exists(path):
         raise MHError(MH_ERR_NODEVICE, f"HIP library not built: {path} (run python -m large_sys_eq_opt_amd.build)")
     lib = C.CDLL(path)

@@ -98,9 +98,14 @@ __device__ void grid_barrier(unsigned* bar, unsigned nblocks, int* err) {
     __syncthreads();
 }
 
+union BatchSlice {   // a wavefront's LDS: x-tile exchange or y-tile ring
+    XLdsC x;
+    double ring[kRing];
+};
+
 template <int S>
 __global__ __launch_bounds__(256) void k_batch_step(BatchArgs a) {
-    __shared__ XLds X[4];
+    __shared__ BatchSlice X[4];
     const int w = threadIdx.x >> 6;
     const int lane = threadIdx.x & 63;
     const int gw = blockIdx.x * 4 + w;
@@ -131,7 +136,7 @@ __global__ __launch_bounds__(256) void k_batch_step(BatchArgs a) {
             double dt;
             if (!inst_dt(I, a.clk[t.inst], par, dt)) continue;
             const double dtd = dt / I.dx;
-            x_tile<S>(I.U, I.T, t.j, t.i0, t.i1, dtd, 0.5 * dtd, I.bc, 1, X[w], lane);
+            x_tile<S, 0>(I.U, I.T, t.j, t.i0, t.i1, dtd, 0.5 * dtd, I.bc, 1, X[w].x, lane);
         }
     }
     if (a.phases == 3) grid_barrier(&a.ctrl->bar[par], gridDim.x, &a.ctrl->err);
@@ -142,7 +147,7 @@ __global__ __launch_bounds__(256) void k_batch_step(BatchArgs a) {
             double dt;
             if (!inst_dt(I, a.clk[t.inst], par, dt)) continue;
             const double dtd = dt / I.dy;
-            double m = y_tile<S>(I.T, I.U, t.cg, t.j0, t.j1, dtd, 0.5 * dtd, I.bc, 1, 1, lane);
+            double m = y_tile<S>(I.T, I.U, t.cg, t.j0, t.j1, dtd, 0.5 * dtd, I.bc, 1, 1, X[w].ring, lane);
             m = wave_max(m);
             if (lane == 0 && m > 0.0)
                 atomicMax(&a.clk[t.inst].smax[par ^ 1], (unsigned long long)__double_as_longlong(m));

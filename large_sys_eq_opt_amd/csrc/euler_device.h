@@ -33,10 +33,12 @@ struct Prim { double r, u, v, p; };     // rho, u, v, p
 
 __device__ __forceinline__ bool finite(double x) { return __builtin_isfinite(x); }
 
-// libm fmin/fmax (x86-64 glibc: the second operand on ties) and std::min/max
-// (the first operand on ties) -- used exactly where the reference uses each.
-__device__ __forceinline__ double lmin(double a, double b) { return (a < b) ? a : b; }
-__device__ __forceinline__ double lmax(double a, double b) { return (a > b) ? a : b; }
+// libm fmin/fmax and std::min/max -- used exactly where the reference uses
+// each.  Every fmin/fmax operand on this path is finite and non-zero ties
+// have equal bits (the operands are sums with a positive sound speed, or the
+// positive PVRS floor), so the hardware v_min/v_max_f64 gives the libm result.
+__device__ __forceinline__ double lmin(double a, double b) { return __builtin_fmin(a, b); }
+__device__ __forceinline__ double lmax(double a, double b) { return __builtin_fmax(a, b); }
 __device__ __forceinline__ double smin(double a, double b) { return (b < a) ? b : a; }
 __device__ __forceinline__ double smax(double a, double b) { return (a < b) ? b : a; }
 
@@ -136,15 +138,16 @@ __device__ __forceinline__ Cons hllc(const Cons& UL, const Prim& VL, const Cons&
     if (fabs(denominator) < 1.0e-14) return hll<D>(UL, VL, UR, VR);
     const double Sstar = numerator / denominator;
     if (!finite(Sstar)) return hll<D>(UL, VL, UR, VR);
-    // One star state is formed, chosen by the sign of S*; both branches are
-    // F_K + S_K (U*_K - U_K) with K = L or R.
-    const bool left = Sstar >= 0.0;
-    const Cons& U = left ? UL : UR;
-    const Prim& V = left ? VL : VR;
-    const Cons& F = left ? FL : FR;
-    const double S = left ? SL : SR;
-    const Cons Us = hllc_star<D>(U, V, S, Sstar);
-    return {F.r + S * (Us.r - U.r), F.mu + S * (Us.mu - U.mu), F.mv + S * (Us.mv - U.mv), F.e + S * (Us.e - U.e)};
+    // One star state is formed, on the side the sign of S* picks
+    // (neighbouring faces mostly agree, so the branch rarely diverges).
+    if (Sstar >= 0.0) {
+        const Cons Us = hllc_star<D>(UL, VL, SL, Sstar);
+        return {FL.r + SL * (Us.r - UL.r), FL.mu + SL * (Us.mu - UL.mu), FL.mv + SL * (Us.mv - UL.mv),
+                FL.e + SL * (Us.e - UL.e)};
+    }
+    const Cons Us = hllc_star<D>(UR, VR, SR, Sstar);
+    return {FR.r + SR * (Us.r - UR.r), FR.mu + SR * (Us.mu - UR.mu), FR.mv + SR * (Us.mv - UR.mv),
+            FR.e + SR * (Us.e - UR.e)};
 }
 
 // ---------------------------------------------------------------------------
@@ -316,9 +319,14 @@ __device__ __forceinline__ Cons riemann(const Cons& UL, const Prim& VL, const Co
 // Outputs the evolved edge states and their primitive states.
 // `half` = 0.5 * dt / d (the reference forms 0.5 * dt_over_d the same way).
 // ---------------------------------------------------------------------------
+// minmod with the reference's operation (a*b <= 0 -> 0; else std::min for
+// a > 0, std::max for a < 0).  For a*b > 0 both branches pick the operand of
+// smaller magnitude, a on ties, and a NaN operand propagates exactly as the
+// reference's std::min/std::max do (a NaN a is kept; a NaN b loses), so the
+// single magnitude compare is bit-identical to it.
 __device__ __forceinline__ double minmod(double a, double b) {
     if (a * b <= 0.0) return 0.0;
-    return (a > 0.0) ? smin(a, b) : smax(a, b);
+    return (fabs(b) < fabs(a)) ? b : a;
 }
 
 struct Edge { Cons U; Prim V; };
@@ -331,17 +339,26 @@ __device__ __forceinline__ void reconstruct(const Prim& Wm, const Prim& Wc, cons
     const Prim hs = {0.5 * s.r, 0.5 * s.u, 0.5 * s.v, 0.5 * s.p};
     Prim WL = {Wc.r - hs.r, Wc.u - hs.u, Wc.v - hs.v, Wc.p - hs.p};
     Prim WR = {Wc.r + hs.r, Wc.u + hs.u, Wc.v + hs.v, Wc.p + hs.p};
-    if (!phys_recon(WL)) WL = Wc;
-    if (!phys_recon(WR)) WR = Wc;
+    // The fallbacks (src/cpu/solver_cpu.cpp:86-87, :97-98) almost never fire:
+    // when every lane of the wavefront passes a check the selects are skipped.
+    const bool okL = phys_recon(WL), okR = phys_recon(WR);
+    if (!__all(okL && okR)) {
+        if (!okL) WL = Wc;
+        if (!okR) WR = Wc;
+    }
     const Cons UL = p2c(WL), UR = p2c(WR);
     const Prim VL = c2p(UL), VR = c2p(UR);
     const Cons FL = pflux<D>(UL, VL), FR = pflux<D>(UR, VR);
     const Cons h = {half * (FR.r - FL.r), half * (FR.mu - FL.mu), half * (FR.mv - FL.mv), half * (FR.e - FL.e)};
-    const Cons cL = {UL.r - h.r, UL.mu - h.mu, UL.mv - h.mv, UL.e - h.e};
-    const Cons cR = {UR.r - h.r, UR.mu - h.mu, UR.mv - h.mv, UR.e - h.e};
-    const Prim pL = c2p(cL), pR = c2p(cR);
-    if (phys_recon(pL)) { L.U = cL; L.V = pL; } else { L.U = UL; L.V = VL; }
-    if (phys_recon(pR)) { R.U = cR; R.V = pR; } else { R.U = UR; R.V = VR; }
+    L.U = {UL.r - h.r, UL.mu - h.mu, UL.mv - h.mv, UL.e - h.e};
+    R.U = {UR.r - h.r, UR.mu - h.mu, UR.mv - h.mv, UR.e - h.e};
+    L.V = c2p(L.U);
+    R.V = c2p(R.U);
+    const bool eL = phys_recon(L.V), eR = phys_recon(R.V);
+    if (!__all(eL && eR)) {
+        if (!eL) { L.U = UL; L.V = VL; }
+        if (!eR) { R.U = UR; R.V = VR; }
+    }
 }
 
 // Cell update U - (dt/d) * (Fp - Fm)  (src/cpu/solver_cpu.cpp:452-453).

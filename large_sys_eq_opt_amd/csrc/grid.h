@@ -13,7 +13,7 @@
 #include <cstdint>
 #include <string>
 
-#include "../../include/mh_euler.h"
+#include "mh_euler.h"
 
 namespace mh {
 

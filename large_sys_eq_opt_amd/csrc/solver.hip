@@ -88,6 +88,7 @@ struct SweepArgs {
     int seg;           // rows (y) or cells (x) per wavefront
     int ngroups;       // column groups (y) or segments per row (x)
     int ghosts;        // write the fused ghost layers of the output
+    int ntiles;        // wavefront tiles in the launch
     Bc bc;
     // y sweep only: where the wavefront max of the output's signal speed
     // goes -- a fixed slot, or (clock_speed) the clock's inactive slot.
@@ -95,39 +96,52 @@ struct SweepArgs {
     int clock_speed;
 };
 
-template <int S>
-__global__ __launch_bounds__(64) void k_sweep_y(View in, View out, SweepArgs a) {
+// Sweep kernels: WPB independent wavefronts per workgroup, one tile each
+// (wavefront-level synchronisation only).  For the y sweep, W is the
+// wavefronts per SIMD the register allocation aims for (1 = compiler's
+// choice; four fit the HLL/HLLC/FORCE y tile in 128 VGPRs).  For the x
+// sweep, L selects the LDS exchange layout.
+template <int S, int W, int WPB>
+__global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(W, 8))) void k_sweep_y(View in, View out,
+                                                                                               SweepArgs a) {
     double dt;
     if (!dt_of(a.dts, dt)) return;
     const double dtd = dt / a.d;
     const double half = 0.5 * dtd;
-    const int cg = blockIdx.x % a.ngroups;
-    const int ch = blockIdx.x / a.ngroups;
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int tile = blockIdx.x * WPB + w;
+    if (tile >= a.ntiles) return;
+    const int cg = tile % a.ngroups;
+    const int ch = tile / a.ngroups;
     const int j0 = ch * a.seg;
     const int j1 = min(j0 + a.seg, in.ny);
     const int want = (a.speed || a.clock_speed) ? 1 : 0;
-    double m = y_tile<S>(in, out, cg, j0, j1, dtd, half, a.bc, a.ghosts, want, threadIdx.x);
+    __shared__ double ring[WPB][kRing];
+    double m = y_tile<S>(in, out, cg, j0, j1, dtd, half, a.bc, a.ghosts, want, ring[w], lane);
     if (want) {
         m = wave_max(m);
-        if (threadIdx.x == 0) {
+        if (lane == 0) {
             unsigned long long* slot = a.clock_speed ? &a.dts.clock->smax[a.dts.clock->slot ^ 1] : a.speed;
             atomic_max_pos(slot, m);
         }
     }
 }
 
-template <int S>
-__global__ __launch_bounds__(64) void k_sweep_x(View in, View out, SweepArgs a) {
+template <int S, int L, int WPB>
+__global__ __launch_bounds__(64 * WPB) void k_sweep_x(View in, View out, SweepArgs a) {
     double dt;
     if (!dt_of(a.dts, dt)) return;
     const double dtd = dt / a.d;
     const double half = 0.5 * dtd;
-    const int j = blockIdx.x / a.ngroups;
-    const int sg = blockIdx.x - j * a.ngroups;
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int tile = blockIdx.x * WPB + w;
+    if (tile >= a.ntiles) return;
+    const int j = tile / a.ngroups;
+    const int sg = tile - j * a.ngroups;
     const int i0 = sg * a.seg;
     const int i1 = min(i0 + a.seg, in.nx);
-    __shared__ XLds X;
-    x_tile<S>(in, out, j, i0, i1, dtd, half, a.bc, a.ghosts, X, threadIdx.x);
+    __shared__ typename XSlice<L>::type X[WPB];
+    x_tile<S, L>(in, out, j, i0, i1, dtd, half, a.bc, a.ghosts, X[w], lane);
 }
 
 // ---------------------------------------------------------------------------
@@ -302,6 +316,12 @@ __global__ __launch_bounds__(256) void k_recon(int n, const double* Um, const do
     R[4 * k] = r.U.r; R[4 * k + 1] = r.U.mu; R[4 * k + 2] = r.U.mv; R[4 * k + 3] = r.U.e;
 }
 
+// Plain streaming copy (calibrates traffic counters; measured copy peak).
+__global__ __launch_bounds__(256) void k_copy(const double2* __restrict__ src, double2* __restrict__ dst, int64_t n2) {
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n2; k += (int64_t)gridDim.x * blockDim.x)
+        dst[k] = src[k];
+}
+
 // ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
@@ -322,11 +342,32 @@ int valid_bc(const mh_bc* b) {
 }
 
 // Sweep launch shapes (the autotuner may override them).
-struct Shape {
-    int seg_x = 0;    // 0 = whole row per wavefront
-    int seg_y = 256;  // rows per wavefront chunk
-};
-static Shape g_shape;
+// The defaults keep the number of wavefronts well above the resident slots
+// for large grids (whole rows and 256-row chunks leave a second, partly
+// filled round of wavefronts at 8192^2); 1020 = 16 x 64 - 4 cells fill the
+// x tile's 64-lane iterations exactly.
+// Defaults measured fastest at 8192^2 (HLLC): see the launch-shape sweep in
+// the design notes; the autotuner can override them.
+static mh_launch_shape g_shape = {1020, 128, 1, 0, 1, 4};
+
+template <int S, int L, int WPB>
+void go_x(dim3 g, hipStream_t st, const View& vi, const View& vo, const SweepArgs& a) {
+    hipLaunchKernelGGL((k_sweep_x<S, L, WPB>), g, dim3(64 * WPB), 0, st, vi, vo, a);
+}
+template <int S, int W, int WPB>
+void go_y(dim3 g, hipStream_t st, const View& vi, const View& vo, const SweepArgs& a) {
+    hipLaunchKernelGGL((k_sweep_y<S, W, WPB>), g, dim3(64 * WPB), 0, st, vi, vo, a);
+}
+template <int S>
+void go_x_s(int layout, int wpb, dim3 g, hipStream_t st, const View& vi, const View& vo, const SweepArgs& a) {
+    if (layout == 0) (wpb == 4 ? go_x<S, 0, 4> : go_x<S, 0, 1>)(g, st, vi, vo, a);
+    else (wpb == 4 ? go_x<S, 1, 4> : go_x<S, 1, 1>)(g, st, vi, vo, a);
+}
+template <int S>
+void go_y_s(int waves, int wpb, dim3 g, hipStream_t st, const View& vi, const View& vo, const SweepArgs& a) {
+    if (S == kExact || waves != 4) (wpb == 4 ? go_y<S, 1, 4> : go_y<S, 1, 1>)(g, st, vi, vo, a);
+    else (wpb == 4 ? go_y<S, 4, 4> : go_y<S, 4, 1>)(g, st, vi, vo, a);
+}
 
 int launch_sweep(const mh_grid_s* in, const mh_grid_s* out, int dir, int solver, DtSrc dts, const Bc& bc, int ghosts,
                  unsigned long long* speed, int clock_speed, hipStream_t st) {
@@ -341,24 +382,29 @@ int launch_sweep(const mh_grid_s* in, const mh_grid_s* out, int dir, int solver,
         a.d = in->dx;
         a.seg = g_shape.seg_x > 0 ? g_shape.seg_x : in->d.nx;
         a.ngroups = (in->d.nx + a.seg - 1) / a.seg;
-        const dim3 grid((unsigned)((int64_t)a.ngroups * in->d.ny));
+        a.ntiles = a.ngroups * in->d.ny;
+        const int wpb = g_shape.x_wpb;
+        const dim3 g((unsigned)((a.ntiles + wpb - 1) / wpb));
+        const int L = g_shape.x_layout;
         switch (solver) {
-            case kHLL: hipLaunchKernelGGL(k_sweep_x<kHLL>, grid, dim3(64), 0, st, vi, vo, a); break;
-            case kHLLC: hipLaunchKernelGGL(k_sweep_x<kHLLC>, grid, dim3(64), 0, st, vi, vo, a); break;
-            case kExact: hipLaunchKernelGGL(k_sweep_x<kExact>, grid, dim3(64), 0, st, vi, vo, a); break;
-            default: hipLaunchKernelGGL(k_sweep_x<kForce>, grid, dim3(64), 0, st, vi, vo, a); break;
+            case kHLL: go_x_s<kHLL>(L, wpb, g, st, vi, vo, a); break;
+            case kHLLC: go_x_s<kHLLC>(L, wpb, g, st, vi, vo, a); break;
+            case kExact: go_x_s<kExact>(L, wpb, g, st, vi, vo, a); break;
+            default: go_x_s<kForce>(L, wpb, g, st, vi, vo, a); break;
         }
     } else {
         a.d = in->dy;
         a.seg = g_shape.seg_y > 0 ? g_shape.seg_y : in->d.ny;
         a.ngroups = (in->d.nx + 63) / 64;
-        const int nch = (in->d.ny + a.seg - 1) / a.seg;
-        const dim3 grid((unsigned)((int64_t)a.ngroups * nch));
+        a.ntiles = a.ngroups * ((in->d.ny + a.seg - 1) / a.seg);
+        const int wpb = g_shape.y_wpb;
+        const dim3 g((unsigned)((a.ntiles + wpb - 1) / wpb));
+        const int W = g_shape.y_waves;
         switch (solver) {
-            case kHLL: hipLaunchKernelGGL(k_sweep_y<kHLL>, grid, dim3(64), 0, st, vi, vo, a); break;
-            case kHLLC: hipLaunchKernelGGL(k_sweep_y<kHLLC>, grid, dim3(64), 0, st, vi, vo, a); break;
-            case kExact: hipLaunchKernelGGL(k_sweep_y<kExact>, grid, dim3(64), 0, st, vi, vo, a); break;
-            default: hipLaunchKernelGGL(k_sweep_y<kForce>, grid, dim3(64), 0, st, vi, vo, a); break;
+            case kHLL: go_y_s<kHLL>(W, wpb, g, st, vi, vo, a); break;
+            case kHLLC: go_y_s<kHLLC>(W, wpb, g, st, vi, vo, a); break;
+            case kExact: go_y_s<kExact>(W, wpb, g, st, vi, vo, a); break;
+            default: go_y_s<kForce>(W, wpb, g, st, vi, vo, a); break;
         }
     }
     return check_hip(hipGetLastError(), "sweep launch");
@@ -837,13 +883,43 @@ int mh_stepper_advance_profiled(mh_stepper s, int nsteps, mh_stream stream, doub
     return MH_OK;
 }
 
-int mh_set_launch_shape(int seg_x, int seg_y) {
-    if (seg_x < 0 || seg_y < 0) return fail(MH_ERR_INVALID, "mh_set_launch_shape: negative segment");
-    g_shape.seg_x = seg_x;
-    g_shape.seg_y = seg_y;
+int mh_set_launch_shape(const mh_launch_shape* s) {
+    if (!s) return fail(MH_ERR_INVALID, "mh_set_launch_shape: null shape");
+    if (s->seg_x < 0 || s->seg_y < 0) return fail(MH_ERR_INVALID, "mh_set_launch_shape: negative segment");
+    if (s->seg_x > 0 && s->seg_x < 1) return fail(MH_ERR_INVALID, "mh_set_launch_shape: bad segment");
+    if (s->y_waves != 1 && s->y_waves != 4) return fail(MH_ERR_INVALID, "mh_set_launch_shape: y_waves must be 1 or 4");
+    if (s->x_layout != 0 && s->x_layout != 1) return fail(MH_ERR_INVALID, "mh_set_launch_shape: x_layout must be 0 or 1");
+    if ((s->x_wpb != 1 && s->x_wpb != 4) || (s->y_wpb != 1 && s->y_wpb != 4))
+        return fail(MH_ERR_INVALID, "mh_set_launch_shape: wavefronts per workgroup must be 1 or 4");
+    g_shape = *s;
     return MH_OK;
 }
 
+int mh_get_launch_shape(mh_launch_shape* s) {
+    if (!s) return fail(MH_ERR_INVALID, "mh_get_launch_shape: null shape");
+    *s = g_shape;
+    return MH_OK;
+}
+
+int mh_copy(const double* src, double* dst, int64_t n, mh_stream stream) {
+    if (!src || !dst || n < 0 || (n & 1)) return fail(MH_ERR_INVALID, "mh_copy: need device buffers and an even n");
+    const int64_t n2 = n / 2;
+    const int blocks = (int)std::min<int64_t>((n2 + 255) / 256, 256 * 32);
+    hipLaunchKernelGGL(k_copy, dim3(std::max(blocks, 1)), dim3(256), 0, (hipStream_t)stream, (const double2*)src,
+                       (double2*)dst, n2);
+    return check_hip(hipGetLastError(), "copy launch");
+}
+
+int mh_device_alloc(int64_t bytes, void** ptr) {
+    if (!ptr || bytes <= 0) return fail(MH_ERR_INVALID, "mh_device_alloc: bad argument");
+    return check_hip(hipMalloc(ptr, (size_t)bytes), "hipMalloc");
+}
+
+int mh_device_free(void* ptr) { return check_hip(hipFree(ptr), "hipFree"); }
+
+int mh_synchronize(mh_stream stream) { return check_hip(hipStreamSynchronize((hipStream_t)stream), "hipStreamSynchronize"); }
+
 }  // extern "C"
+
 
 

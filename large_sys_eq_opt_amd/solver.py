@@ -207,9 +207,22 @@ def sweep_gpu(Uin, Uout, dt, direction, solver, stream=None):
     call("mh_sweep", Uin.handle, Uout.handle, float(dt), int(direction), int(solver), _stream(stream))
 
 
-def set_launch_shape(seg_x: int, seg_y: int):
-    """x-sweep row segment (0 = whole row) and y-sweep chunk rows (0 = whole column)."""
-    call("mh_set_launch_shape", int(seg_x), int(seg_y))
+SHAPE_FIELDS = ("seg_x", "seg_y", "y_waves", "x_layout", "x_wpb", "y_wpb")
+
+
+def set_launch_shape(**kw):
+    """Sets fields of the sweeps' launch shape (see mh_launch_shape):
+    seg_x, seg_y, y_waves, x_layout, x_wpb, y_wpb."""
+    cur = get_launch_shape()
+    cur.update({k: int(v) for k, v in kw.items()})
+    shape = _lib.LaunchShape(*(cur[f] for f in SHAPE_FIELDS))
+    call("mh_set_launch_shape", C.byref(shape))
+
+
+def get_launch_shape() -> dict:
+    shape = _lib.LaunchShape()
+    call("mh_get_launch_shape", C.byref(shape))
+    return {f: getattr(shape, f) for f in SHAPE_FIELDS}
 
 
 def apply_boundary_gpu(grid: Grid2DGPU, bc: Boundary | None = None, stream=None):
