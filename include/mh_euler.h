@@ -152,6 +152,10 @@ typedef struct {
     int x_layout; /* x tile LDS exchange: 0 compact (6.3 KB, selects), 1 66-slot (10.3 KB) */
     int x_wpb;    /* wavefronts (tiles) per workgroup, x sweep: 1 or 4 */
     int y_wpb;    /* wavefronts (tiles) per workgroup, y sweep: 1 or 4 */
+    int min_tiles; /* a sweep whose segments give fewer tiles than this gets
+                      shorter ones (x: multiples of 60 cells, at least 60;
+                      y: at least 4 rows) and, while still below it, one
+                      tile per workgroup; 0 keeps the shape as set */
 } mh_launch_shape;
 int mh_set_launch_shape(const mh_launch_shape* shape);
 int mh_get_launch_shape(mh_launch_shape* shape);
@@ -190,6 +194,35 @@ int mh_batch_capture(mh_batch b, int steps_per_graph, mh_stream stream);
 int mh_batch_clock(mh_batch b, double* t, double* dt, int* steps, int* status, mh_stream stream);
 /* Persistent grid size and tile counts of the current schedule. */
 int mh_batch_info(mh_batch b, int* nblocks, int* nxtiles, int* nytiles);
+
+/* ---- discrete adjoint (no reference counterpart): reverse mode of the
+ *      second-order split step at a fixed dt (HLL and HLLC), with
+ *      checkpointed forward/backward over N steps. ------------------------ */
+/* Workspace for grids of shape desc: checkpoints every checkpoint_every
+ * steps for up to max_steps steps. */
+int mh_adjoint_create(const mh_grid_desc* desc, int solver, const mh_bc* bc, int checkpoint_every, int max_steps,
+                      mh_adjoint* out);
+int mh_adjoint_destroy(mh_adjoint a);
+/* lam_out = (dU^{n+1}/dU^n)^T lam_in for one split step at state U and time
+ * step dt (interior adjoints; U's ghosts are filled by the call). */
+int mh_adjoint_step_vjp(mh_adjoint a, mh_grid U, double dt, mh_grid lam_in, mh_grid lam_out, mh_stream stream);
+/* The same for one sweep alone (dir 0 = x, 1 = y): lam_out covers every
+ * cell the sweep reads, ghost layers included (no boundary folding). */
+int mh_adjoint_sweep_vjp(mh_adjoint a, mh_grid U, double dt, int dir, mh_grid lam_in, mh_grid lam_out,
+                         mh_stream stream);
+/* nsteps steps from U0 (unchanged) with CFL dt (or the sequence dt_in),
+ * checkpointing; dt_out (nsteps, may be NULL) receives the dt sequence, UN
+ * (may be NULL) the final state.  Synchronises. */
+int mh_adjoint_forward(mh_adjoint a, mh_grid U0, int nsteps, double cfl, const double* dt_in, double* dt_out,
+                       mh_grid UN, mh_stream stream);
+/* lam0 = dJ/dU^0 from lamN = dJ/dU^N over the steps of the last forward call
+ * (each checkpoint segment recomputed once).  Synchronises. */
+int mh_adjoint_backward(mh_adjoint a, mh_grid lamN, mh_grid lam0, mh_stream stream);
+/* dt sequence and step count of the last forward call. */
+int mh_adjoint_dt(mh_adjoint a, double* dt, int* nsteps);
+/* Zero every cell of a grid / copy every cell between equal-shape grids. */
+int mh_grid_zero(mh_grid g, mh_stream stream);
+int mh_grid_copy(mh_grid src, mh_grid dst, mh_stream stream);
 
 /* ---- utilities (no reference counterpart) -------------------------------- */
 /* Plain streaming copy of n doubles (n even) between device buffers: the

@@ -10,6 +10,6 @@ from .cases import (CaseConfig, CaseId, case_boundary, get_case_config, get_n_ca
                     initial_state_at, make_initial_grid, make_n_grid, parse_case_id)
 from .solver import (Boundary, GpuWorkspace, Grid2DGPU, RiemannSolver, Stepper,  # noqa: F401
                      advance_first_order_gpu, advance_second_order_gpu, apply_boundary_gpu,
-                     apply_transmissive_boundary_gpu, compute_dt_gpu, parse_riemann_solver, sweep_gpu)
+                     apply_transmissive_boundary_gpu, compute_dt_gpu, parse_riemann_solver, sweep_gpu, synchronize)
 
 __version__ = "1.0.0"

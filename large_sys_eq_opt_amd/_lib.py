@@ -29,7 +29,8 @@ class GridDesc(C.Structure):
 
 
 class LaunchShape(C.Structure):
-    _fields_ = [("seg_x", _I), ("seg_y", _I), ("y_waves", _I), ("x_layout", _I), ("x_wpb", _I), ("y_wpb", _I)]
+    _fields_ = [("seg_x", _I), ("seg_y", _I), ("y_waves", _I), ("x_layout", _I), ("x_wpb", _I), ("y_wpb", _I),
+                ("min_tiles", _I)]
 
 
 class BcDesc(C.Structure):
@@ -75,6 +76,15 @@ SIGNATURES = {
     "mh_batch_capture": ([_P, _I, _P], _I),
     "mh_batch_clock": ([_P, _P, _P, _P, _P, _P], _I),
     "mh_batch_info": ([_P, C.POINTER(_I), C.POINTER(_I), C.POINTER(_I)], _I),
+    "mh_adjoint_create": ([C.POINTER(GridDesc), _I, C.POINTER(BcDesc), _I, _I, C.POINTER(_P)], _I),
+    "mh_adjoint_destroy": ([_P], _I),
+    "mh_adjoint_step_vjp": ([_P, _P, _D, _P, _P, _P], _I),
+    "mh_adjoint_sweep_vjp": ([_P, _P, _D, _I, _P, _P, _P], _I),
+    "mh_adjoint_forward": ([_P, _P, _I, _D, _P, _P, _P, _P], _I),
+    "mh_adjoint_backward": ([_P, _P, _P, _P], _I),
+    "mh_adjoint_dt": ([_P, _P, C.POINTER(_I)], _I),
+    "mh_grid_zero": ([_P, _P], _I),
+    "mh_grid_copy": ([_P, _P, _P], _I),
     "mh_copy": ([_P, _P, _I64, _P], _I),
     "mh_device_alloc": ([_I64, C.POINTER(_P)], _I),
     "mh_device_free": ([_P], _I),
@@ -95,6 +105,14 @@ def load(path: str | None = None):
     path = path or os.environ.get("MH_EULER_LIB") or LIB_PATH
     if not os.path.exists(path):
         raise MHError(MH_ERR_NODEVICE, f"HIP library not built: {path} (run python -m large_sys_eq_opt_amd.build)")
+    # One HIP runtime per process: torch ships its own libamdhip64 (soname
+    # libamdhip64.so.7, loaded by the name libamdhip64.so).  Loaded first, it
+    # is the runtime this library binds to; loaded after this library it
+    # would come in as a second runtime that finds no GPU.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = C.CDLL(path)
     for name, (args, res) in SIGNATURES.items():
         fn = getattr(lib, name)

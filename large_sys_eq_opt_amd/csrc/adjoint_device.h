@@ -1,0 +1,321 @@
+// adjoint_device.h -- hand-written reverse mode (vector-Jacobian products)
+// of the device physics in euler_device.h: conversions, physical flux, sound
+// speed, HLL and HLLC with every branch, and the MUSCL-Hancock
+// reconstruction with its limiter and fallbacks.
+//
+// Conventions.  `b` prefixes an adjoint; every *_vjp accumulates (+=) into
+// the adjoints of its inputs.  A primitive state passed next to its
+// conserved state is always c2p of it, and its adjoint is returned
+// separately: the caller folds it into the conserved adjoint with c2p_vjp.
+// Branches are those the primal takes; a non-smooth point takes a fixed
+// one-sided derivative: minmod picks the operand it returns (a on ties,
+// zero when a*b <= 0), fmin/fmax the operand they return (the second on
+// ties).  The time step is a constant (the adjoint of a split step at a
+// given dt).
+#pragma once
+
+#include "euler_device.h"
+
+namespace mh {
+
+__device__ __forceinline__ void acc(Cons& a, double r, double mu, double mv, double e) {
+    a.r += r; a.mu += mu; a.mv += mv; a.e += e;
+}
+__device__ __forceinline__ void acc(Prim& a, double r, double u, double v, double p) {
+    a.r += r; a.u += u; a.v += v; a.p += p;
+}
+__device__ __forceinline__ Cons czero() { return {0.0, 0.0, 0.0, 0.0}; }
+__device__ __forceinline__ Prim pzero() { return {0.0, 0.0, 0.0, 0.0}; }
+
+// V = c2p(U): u = mu/r, v = mv/r, p = (g-1)(e - r(u^2+v^2)/2).
+__device__ __forceinline__ void c2p_vjp(const Cons& U, const Prim& bV, Cons& bU) {
+    const double inv = 1.0 / U.r;
+    const double u = U.mu * inv, v = U.mv * inv;
+    const double h = 0.5 * U.r, s = u * u + v * v;
+    const double bkin = -kGm1 * bV.p;
+    const double bs = h * bkin;
+    const double bu = bV.u + 2.0 * u * bs, bv = bV.v + 2.0 * v * bs;
+    const double binv = U.mu * bu + U.mv * bv;
+    bU.r += bV.r + 0.5 * s * bkin - inv * inv * binv;
+    bU.mu += inv * bu;
+    bU.mv += inv * bv;
+    bU.e += kGm1 * bV.p;
+}
+
+// U = p2c(V): (r, r u, r v, p/(g-1) + r(u^2+v^2)/2).
+__device__ __forceinline__ void p2c_vjp(const Prim& V, const Cons& bU, Prim& bV) {
+    bV.r += bU.r + V.u * bU.mu + V.v * bU.mv + 0.5 * (V.u * V.u + V.v * V.v) * bU.e;
+    bV.u += V.r * bU.mu + V.r * V.u * bU.e;
+    bV.v += V.r * bU.mv + V.r * V.v * bU.e;
+    bV.p += bU.e / kGm1;
+}
+
+template <int D>
+__device__ __forceinline__ void pflux_vjp(const Cons& U, const Prim& V, const Cons& bF, Cons& bU, Prim& bV) {
+    if (D == kX) {   // (mu, mu u + p, mv u, (e+p) u)
+        bU.mu += bF.r + V.u * bF.mu;
+        bU.mv += V.u * bF.mv;
+        bU.e += V.u * bF.e;
+        bV.u += U.mu * bF.mu + U.mv * bF.mv + (U.e + V.p) * bF.e;
+        bV.p += bF.mu + V.u * bF.e;
+    } else {         // (mv, mu v, mv v + p, (e+p) v)
+        bU.mv += bF.r + V.v * bF.mv;
+        bU.mu += V.v * bF.mu;
+        bU.e += V.v * bF.e;
+        bV.v += U.mu * bF.mu + U.mv * bF.mv + (U.e + V.p) * bF.e;
+        bV.p += bF.mv + V.v * bF.e;
+    }
+}
+
+// a = sqrt(g p / r)
+__device__ __forceinline__ void sound_vjp(const Prim& V, double a, double ba, Prim& bV) {
+    const double s = kGamma * V.p / V.r;
+    const double bs = ba / (2.0 * a);
+    bV.p += kGamma / V.r * bs;
+    bV.r -= s / V.r * bs;
+}
+
+template <int D> __device__ __forceinline__ void add_un(Prim& b, double x) { if (D == kX) b.u += x; else b.v += x; }
+template <int D> __device__ __forceinline__ void add_ut(Prim& b, double x) { if (D == kX) b.v += x; else b.u += x; }
+
+// S_L = fmin(unL - aL, unR - aR), S_R = fmax(unL + aL, unR + aR)
+template <int D>
+__device__ __forceinline__ void speeds_vjp(const Prim& VL, const Prim& VR, double aL, double aR, double bSL,
+                                           double bSR, Prim& bVL, Prim& bVR) {
+    const double unL = un_of<D>(VL), unR = un_of<D>(VR);
+    double baL = 0.0, baR = 0.0;
+    if ((unL - aL) < (unR - aR)) { add_un<D>(bVL, bSL); baL -= bSL; }
+    else { add_un<D>(bVR, bSL); baR -= bSL; }
+    if ((unL + aL) > (unR + aR)) { add_un<D>(bVL, bSR); baL += bSR; }
+    else { add_un<D>(bVR, bSR); baR += bSR; }
+    sound_vjp(VL, aL, baL, bVL);
+    sound_vjp(VR, aR, baR, bVR);
+}
+
+// ---------------------------------------------------------------------------
+// HLL (head/riemann.hpp:87-129)
+// ---------------------------------------------------------------------------
+template <int D>
+__device__ void hll_vjp(const Cons& UL, const Prim& VL, const Cons& UR, const Prim& VR, const Cons& bF, Cons& bUL,
+                        Prim& bVL, Cons& bUR, Prim& bVR) {
+    const Cons FL = pflux<D>(UL, VL), FR = pflux<D>(UR, VR);
+    if (!phys_riemann(VL) || !phys_riemann(VR)) {
+        const Cons h = {0.5 * bF.r, 0.5 * bF.mu, 0.5 * bF.mv, 0.5 * bF.e};
+        pflux_vjp<D>(UL, VL, h, bUL, bVL);
+        pflux_vjp<D>(UR, VR, h, bUR, bVR);
+        return;
+    }
+    const double aL = sound(VL), aR = sound(VR);
+    const double unL = un_of<D>(VL), unR = un_of<D>(VR);
+    const double SL = lmin(unL - aL, unR - aR);
+    const double SR = lmax(unL + aL, unR + aR);
+    if (SL >= 0.0) { pflux_vjp<D>(UL, VL, bF, bUL, bVL); return; }
+    if (SR <= 0.0) { pflux_vjp<D>(UR, VR, bF, bUR, bVR); return; }
+    const double denom = SR - SL;
+    if (fabs(denom) < 1.0e-14) {
+        const Cons h = {0.5 * bF.r, 0.5 * bF.mu, 0.5 * bF.mv, 0.5 * bF.e};
+        pflux_vjp<D>(UL, VL, h, bUL, bVL);
+        pflux_vjp<D>(UR, VR, h, bUR, bVR);
+        return;
+    }
+    const double sls = SL * SR;
+    const double fl[4] = {FL.r, FL.mu, FL.mv, FL.e}, fr[4] = {FR.r, FR.mu, FR.mv, FR.e};
+    const double ul[4] = {UL.r, UL.mu, UL.mv, UL.e}, ur[4] = {UR.r, UR.mu, UR.mv, UR.e};
+    const double bf[4] = {bF.r, bF.mu, bF.mv, bF.e};
+    double bSL = 0.0, bSR = 0.0, bden = 0.0, bfl[4], bfr[4], bul[4], bur[4];
+    for (int c = 0; c < 4; ++c) {
+        const double N = SR * fl[c] - SL * fr[c] + sls * (ur[c] - ul[c]);
+        const double bN = bf[c] / denom;
+        bden -= (N / denom) * bf[c] / denom;
+        bSR += fl[c] * bN + SL * (ur[c] - ul[c]) * bN;
+        bSL += -fr[c] * bN + SR * (ur[c] - ul[c]) * bN;
+        bfl[c] = SR * bN;
+        bfr[c] = -SL * bN;
+        bur[c] = sls * bN;
+        bul[c] = -sls * bN;
+    }
+    bSR += bden;
+    bSL -= bden;
+    acc(bUL, bul[0], bul[1], bul[2], bul[3]);
+    acc(bUR, bur[0], bur[1], bur[2], bur[3]);
+    pflux_vjp<D>(UL, VL, Cons{bfl[0], bfl[1], bfl[2], bfl[3]}, bUL, bVL);
+    pflux_vjp<D>(UR, VR, Cons{bfr[0], bfr[1], bfr[2], bfr[3]}, bUR, bVR);
+    speeds_vjp<D>(VL, VR, aL, aR, bSL, bSR, bVL, bVR);
+}
+
+// ---------------------------------------------------------------------------
+// HLLC (head/riemann.hpp:135-271)
+// ---------------------------------------------------------------------------
+template <int D>
+__device__ void hllc_vjp(const Cons& UL, const Prim& VL, const Cons& UR, const Prim& VR, const Cons& bF, Cons& bUL,
+                         Prim& bVL, Cons& bUR, Prim& bVR) {
+    if (!phys_riemann(VL) || !phys_riemann(VR)) { hll_vjp<D>(UL, VL, UR, VR, bF, bUL, bVL, bUR, bVR); return; }
+    const double aL = sound(VL), aR = sound(VR);
+    const double unL = un_of<D>(VL), unR = un_of<D>(VR);
+    const double pL = VL.p, pR = VR.p, rhoL = VL.r, rhoR = VR.r;
+    const double SL = lmin(unL - aL, unR - aR);
+    const double SR = lmax(unL + aL, unR + aR);
+    if (SL >= 0.0) { pflux_vjp<D>(UL, VL, bF, bUL, bVL); return; }
+    if (SR <= 0.0) { pflux_vjp<D>(UR, VR, bF, bUR, bVR); return; }
+    const double numerator = pR - pL + rhoL * unL * (SL - unL) - rhoR * unR * (SR - unR);
+    const double denominator = rhoL * (SL - unL) - rhoR * (SR - unR);
+    if (fabs(denominator) < 1.0e-14) { hll_vjp<D>(UL, VL, UR, VR, bF, bUL, bVL, bUR, bVR); return; }
+    const double Ss = numerator / denominator;
+    if (!finite(Ss)) { hll_vjp<D>(UL, VL, UR, VR, bF, bUL, bVL, bUR, bVR); return; }
+    const bool left = Ss >= 0.0;
+    const Cons& U = left ? UL : UR;
+    const Prim& V = left ? VL : VR;
+    const double S = left ? SL : SR;
+    Cons& bU = left ? bUL : bUR;
+    Prim& bV = left ? bVL : bVR;
+    // star state (hllc_star) forward
+    const double rho = V.r, p = V.p, un = un_of<D>(V), ut = ut_of<D>(V);
+    const double dS = S - Ss;
+    bool star = fabs(dS) >= 1.0e-14;
+    double rs = 0.0, Epr = 0.0, Q = 0.0, P = 0.0, A = 0.0, Bq = 0.0, inner = 0.0, Es = 0.0;
+    if (star) {
+        Q = rho * (S - un);
+        rs = Q / dS;
+        star = finite(rs) && rs > 0.0;
+        if (star) {
+            Epr = U.e / rho;
+            P = p / Q;
+            A = Ss - un;
+            Bq = Ss + P;
+            inner = Epr + A * Bq;
+            Es = rs * inner;
+            star = finite(Es);
+        }
+    }
+    Cons Us = U;
+    if (star) {
+        Us.r = rs;
+        Us.e = Es;
+        if (D == kX) { Us.mu = rs * Ss; Us.mv = rs * ut; } else { Us.mu = rs * ut; Us.mv = rs * Ss; }
+    }
+    // F = F_K + S (U*_K - U_K)
+    const Cons FK = pflux<D>(U, V);
+    pflux_vjp<D>(U, V, bF, bU, bV);
+    double bS = bF.r * (Us.r - U.r) + bF.mu * (Us.mu - U.mu) + bF.mv * (Us.mv - U.mv) + bF.e * (Us.e - U.e);
+    (void)FK;
+    acc(bU, -S * bF.r, -S * bF.mu, -S * bF.mv, -S * bF.e);
+    const Cons bUs = {S * bF.r, S * bF.mu, S * bF.mv, S * bF.e};
+    double bSs = 0.0;
+    if (!star) {
+        acc(bU, bUs.r, bUs.mu, bUs.mv, bUs.e);
+    } else {
+        const double bn = D == kX ? bUs.mu : bUs.mv;   // momentum along the face normal
+        const double bt = D == kX ? bUs.mv : bUs.mu;
+        double brs = bUs.r + Ss * bn + ut * bt;
+        bSs += rs * bn;
+        const double but = rs * bt;
+        const double bEs = bUs.e;
+        brs += inner * bEs;
+        const double binner = rs * bEs;
+        const double bEpr = binner, bA = Bq * binner, bBq = A * binner;
+        double bun = 0.0, brho = 0.0, bp = 0.0, be = 0.0, bQ = 0.0;
+        bSs += bA + bBq;
+        bun -= bA;
+        bp += bBq / Q;
+        bQ -= P / Q * bBq;
+        be += bEpr / rho;
+        brho -= Epr / rho * bEpr;
+        bQ += brs / dS;
+        const double bdS = -rs / dS * brs;
+        brho += (S - un) * bQ;
+        const double bD = rho * bQ;
+        bS += bD;
+        bun -= bD;
+        bS += bdS;
+        bSs -= bdS;
+        bV.r += brho;
+        bV.p += bp;
+        add_un<D>(bV, bun);
+        add_ut<D>(bV, but);
+        bU.e += be;
+    }
+    // S* = num / den
+    const double bnum = bSs / denominator;
+    const double bden = -Ss / denominator * bSs;
+    double bSL = left ? bS : 0.0, bSR = left ? 0.0 : bS;
+    bVR.p += bnum;
+    bVL.p -= bnum;
+    bVL.r += unL * (SL - unL) * bnum + (SL - unL) * bden;
+    add_un<D>(bVL, rhoL * (SL - 2.0 * unL) * bnum - rhoL * bden);
+    bSL += rhoL * unL * bnum + rhoL * bden;
+    bVR.r += -unR * (SR - unR) * bnum - (SR - unR) * bden;
+    add_un<D>(bVR, -rhoR * (SR - 2.0 * unR) * bnum + rhoR * bden);
+    bSR += -rhoR * unR * bnum - rhoR * bden;
+    speeds_vjp<D>(VL, VR, aL, aR, bSL, bSR, bVL, bVR);
+}
+
+template <int S, int D>
+__device__ __forceinline__ void riemann_vjp(const Cons& UL, const Prim& VL, const Cons& UR, const Prim& VR,
+                                            const Cons& bF, Cons& bUL, Prim& bVL, Cons& bUR, Prim& bVR) {
+    if (S == kHLLC) hllc_vjp<D>(UL, VL, UR, VR, bF, bUL, bVL, bUR, bVR);
+    else hll_vjp<D>(UL, VL, UR, VR, bF, bUL, bVL, bUR, bVR);
+}
+
+// ---------------------------------------------------------------------------
+// reconstruction (src/cpu/solver_cpu.cpp:22-99): adjoints of the evolved
+// edge states (their conserved parts, primitive parts already folded in) to
+// the primitive states of the three cells.
+// ---------------------------------------------------------------------------
+template <int D>
+__device__ void reconstruct_vjp(const Prim& Wm, const Prim& Wc, const Prim& Wp, double half, const Cons& bLU,
+                                const Cons& bRU, Prim& bWm, Prim& bWc, Prim& bWp) {
+    const double am[4] = {Wc.r - Wm.r, Wc.u - Wm.u, Wc.v - Wm.v, Wc.p - Wm.p};
+    const double bm[4] = {Wp.r - Wc.r, Wp.u - Wc.u, Wp.v - Wc.v, Wp.p - Wc.p};
+    double sl[4];
+    int pick[4];   // 0: zero slope, 1: a, 2: b
+    for (int c = 0; c < 4; ++c) {
+        if (am[c] * bm[c] <= 0.0) { sl[c] = 0.0; pick[c] = 0; }
+        else if (fabs(bm[c]) < fabs(am[c])) { sl[c] = bm[c]; pick[c] = 2; }
+        else { sl[c] = am[c]; pick[c] = 1; }
+    }
+    const Prim hs = {0.5 * sl[0], 0.5 * sl[1], 0.5 * sl[2], 0.5 * sl[3]};
+    Prim WL = {Wc.r - hs.r, Wc.u - hs.u, Wc.v - hs.v, Wc.p - hs.p};
+    Prim WR = {Wc.r + hs.r, Wc.u + hs.u, Wc.v + hs.v, Wc.p + hs.p};
+    const bool okL = phys_recon(WL), okR = phys_recon(WR);
+    if (!okL) WL = Wc;
+    if (!okR) WR = Wc;
+    const Cons UL = p2c(WL), UR = p2c(WR);
+    const Prim VL = c2p(UL), VR = c2p(UR);
+    const Cons FL = pflux<D>(UL, VL), FR = pflux<D>(UR, VR);
+    const Cons h = {half * (FR.r - FL.r), half * (FR.mu - FL.mu), half * (FR.mv - FL.mv), half * (FR.e - FL.e)};
+    const Cons cL = {UL.r - h.r, UL.mu - h.mu, UL.mv - h.mv, UL.e - h.e};
+    const Cons cR = {UR.r - h.r, UR.mu - h.mu, UR.mv - h.mv, UR.e - h.e};
+    const bool eL = phys_recon(c2p(cL)), eR = phys_recon(c2p(cR));
+    // backward
+    Cons bUL = czero(), bUR = czero(), bh = czero();
+    acc(bUL, bLU.r, bLU.mu, bLU.mv, bLU.e);
+    if (eL) acc(bh, -bLU.r, -bLU.mu, -bLU.mv, -bLU.e);
+    acc(bUR, bRU.r, bRU.mu, bRU.mv, bRU.e);
+    if (eR) acc(bh, -bRU.r, -bRU.mu, -bRU.mv, -bRU.e);
+    const Cons bFR = {half * bh.r, half * bh.mu, half * bh.mv, half * bh.e};
+    const Cons bFL = {-half * bh.r, -half * bh.mu, -half * bh.mv, -half * bh.e};
+    Prim bVL = pzero(), bVR = pzero();
+    pflux_vjp<D>(UL, VL, bFL, bUL, bVL);
+    pflux_vjp<D>(UR, VR, bFR, bUR, bVR);
+    c2p_vjp(UL, bVL, bUL);
+    c2p_vjp(UR, bVR, bUR);
+    Prim bWL = pzero(), bWR = pzero();
+    p2c_vjp(WL, bUL, bWL);
+    p2c_vjp(WR, bUR, bWR);
+    acc(bWc, bWL.r, bWL.u, bWL.v, bWL.p);
+    acc(bWc, bWR.r, bWR.u, bWR.v, bWR.p);
+    double bhs[4] = {0.0, 0.0, 0.0, 0.0};
+    if (okL) { bhs[0] -= bWL.r; bhs[1] -= bWL.u; bhs[2] -= bWL.v; bhs[3] -= bWL.p; }
+    if (okR) { bhs[0] += bWR.r; bhs[1] += bWR.u; bhs[2] += bWR.v; bhs[3] += bWR.p; }
+    double ba[4] = {0.0, 0.0, 0.0, 0.0}, bb[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int c = 0; c < 4; ++c) {
+        const double bs = 0.5 * bhs[c];
+        if (pick[c] == 1) ba[c] = bs;
+        else if (pick[c] == 2) bb[c] = bs;
+    }
+    acc(bWc, ba[0] - bb[0], ba[1] - bb[1], ba[2] - bb[2], ba[3] - bb[3]);
+    acc(bWm, -ba[0], -ba[1], -ba[2], -ba[3]);
+    acc(bWp, bb[0], bb[1], bb[2], bb[3]);
+}
+
+}  // namespace mh

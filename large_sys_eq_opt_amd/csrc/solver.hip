@@ -78,8 +78,13 @@ __device__ __forceinline__ bool dt_of(const DtSrc& s, double& dt) {
     return true;
 }
 
+// Max of positive doubles through their bit patterns (monotone for m > 0).
+// The slot only grows while a kernel runs, so a read that already shows a
+// value >= m makes the atomic unnecessary; a stale smaller read only costs
+// the atomic.
 __device__ __forceinline__ void atomic_max_pos(unsigned long long* slot, double m) {
-    if (m > 0.0) atomicMax(slot, (unsigned long long)__double_as_longlong(m));
+    const unsigned long long b = (unsigned long long)__double_as_longlong(m);
+    if (m > 0.0 && __hip_atomic_load(slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < b) atomicMax(slot, b);
 }
 
 struct SweepArgs {
@@ -176,7 +181,10 @@ __global__ __launch_bounds__(256) void k_max_speed(View v, unsigned long long* s
         m = smax(m, signal_speed(ld(v, i, j)));
     }
     m = wave_max(m);
-    if ((threadIdx.x & 63) == 0) atomic_max_pos(slot, m);
+    __shared__ double part[4];
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) atomic_max_pos(slot, smax(smax(part[0], part[1]), smax(part[2], part[3])));
 }
 
 __global__ void k_zero_u64(unsigned long long* p, int n) {
@@ -345,10 +353,11 @@ int valid_bc(const mh_bc* b) {
 // The defaults keep the number of wavefronts well above the resident slots
 // for large grids (whole rows and 256-row chunks leave a second, partly
 // filled round of wavefronts at 8192^2); 1020 = 16 x 64 - 4 cells fill the
-// x tile's 64-lane iterations exactly.
-// Defaults measured fastest at 8192^2 (HLLC): see the launch-shape sweep in
-// the design notes; the autotuner can override them.
-static mh_launch_shape g_shape = {1020, 128, 1, 0, 1, 4};
+// x tile's 64-lane iterations exactly.  Compact x exchange with one
+// wavefront per workgroup and four y tiles per workgroup at the compiler's
+// register choice measured fastest for HLLC at 8192^2 on MI355X; the
+// autotuner can override every field.
+static mh_launch_shape g_shape = {1020, 128, 1, 0, 1, 4, 2048};
 
 template <int S, int L, int WPB>
 void go_x(dim3 g, hipStream_t st, const View& vi, const View& vo, const SweepArgs& a) {
@@ -369,6 +378,34 @@ void go_y_s(int waves, int wpb, dim3 g, hipStream_t st, const View& vi, const Vi
     else (wpb == 4 ? go_y<S, 4, 4> : go_y<S, 4, 1>)(g, st, vi, vo, a);
 }
 
+// A grid whose configured segments give fewer than min_tiles tiles (default
+// 2048: two wavefronts on each SIMD of 256 CUs) gets shorter segments, one
+// tile per workgroup: x segments stay a multiple of 60 cells (one 64-lane
+// pass with its 4-cell halo), y chunks at least 4 rows (each chunk reads 4
+// warm-up rows).  Such grids are latency-bound: more, shorter tiles finish
+// sooner although they reread more.
+int seg_x_for(const mh_grid_s* g) {
+    const int nx = g->d.nx, ny = g->d.ny, want_tiles = g_shape.min_tiles;
+    int seg = g_shape.seg_x > 0 ? g_shape.seg_x : nx;
+    if ((int64_t)((nx + seg - 1) / seg) * ny < want_tiles && seg > 60) {
+        const int per_row = (want_tiles + ny - 1) / ny;
+        const int want = (nx + per_row - 1) / per_row;
+        seg = std::min(seg, std::max(60, (want + 59) / 60 * 60));
+    }
+    return seg;
+}
+
+int seg_y_for(const mh_grid_s* g) {
+    const int nx = g->d.nx, ny = g->d.ny, want_tiles = g_shape.min_tiles;
+    int seg = g_shape.seg_y > 0 ? g_shape.seg_y : ny;
+    const int ngroups = (nx + 63) / 64;
+    if ((int64_t)ngroups * ((ny + seg - 1) / seg) < want_tiles && seg > 4) {
+        const int chunks = (want_tiles + ngroups - 1) / ngroups;
+        seg = std::min(seg, std::max(4, (ny + chunks - 1) / chunks));
+    }
+    return seg;
+}
+
 int launch_sweep(const mh_grid_s* in, const mh_grid_s* out, int dir, int solver, DtSrc dts, const Bc& bc, int ghosts,
                  unsigned long long* speed, int clock_speed, hipStream_t st) {
     SweepArgs a;
@@ -380,10 +417,10 @@ int launch_sweep(const mh_grid_s* in, const mh_grid_s* out, int dir, int solver,
     const View vi = in->view(), vo = out->view();
     if (dir == kX) {
         a.d = in->dx;
-        a.seg = g_shape.seg_x > 0 ? g_shape.seg_x : in->d.nx;
+        a.seg = seg_x_for(in);
         a.ngroups = (in->d.nx + a.seg - 1) / a.seg;
         a.ntiles = a.ngroups * in->d.ny;
-        const int wpb = g_shape.x_wpb;
+        const int wpb = a.ntiles < g_shape.min_tiles ? 1 : g_shape.x_wpb;
         const dim3 g((unsigned)((a.ntiles + wpb - 1) / wpb));
         const int L = g_shape.x_layout;
         switch (solver) {
@@ -394,10 +431,10 @@ int launch_sweep(const mh_grid_s* in, const mh_grid_s* out, int dir, int solver,
         }
     } else {
         a.d = in->dy;
-        a.seg = g_shape.seg_y > 0 ? g_shape.seg_y : in->d.ny;
+        a.seg = seg_y_for(in);
         a.ngroups = (in->d.nx + 63) / 64;
         a.ntiles = a.ngroups * ((in->d.ny + a.seg - 1) / a.seg);
-        const int wpb = g_shape.y_wpb;
+        const int wpb = a.ntiles < g_shape.min_tiles ? 1 : g_shape.y_wpb;
         const dim3 g((unsigned)((a.ntiles + wpb - 1) / wpb));
         const int W = g_shape.y_waves;
         switch (solver) {
@@ -421,6 +458,23 @@ int launch_max_speed(const mh_grid_s* g, unsigned long long* slot, hipStream_t s
     const int blocks = (int)std::min<int64_t>((n + 255) / 256, 4096);
     hipLaunchKernelGGL(k_max_speed, dim3(blocks), dim3(256), 0, st, g->view(), slot);
     return check_hip(hipGetLastError(), "max speed launch");
+}
+
+int launch_sweep_host(const mh_grid_s* in, const mh_grid_s* out, int dir, int solver, double dt, const Bc& bc,
+                      int ghosts, hipStream_t st) {
+    return launch_sweep(in, out, dir, solver, DtSrc{nullptr, dt}, bc, ghosts, nullptr, 0, st);
+}
+
+// Max signal speed of a grid's interior, synchronising `st`.
+int max_speed_host(const mh_grid_s* g, double* smax, hipStream_t st) {
+    if (!g->aux) return fail(MH_ERR_INVALID, "max speed: grid of a batch");
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, g->aux, 1);
+    int rc = launch_max_speed(g, g->aux, st);
+    unsigned long long bits = 0;
+    if (!rc) rc = check_hip(hipMemcpyAsync(&bits, g->aux, sizeof(bits), hipMemcpyDeviceToHost, st), "hipMemcpyAsync");
+    if (!rc) rc = check_hip(hipStreamSynchronize(st), "hipStreamSynchronize");
+    if (!rc) std::memcpy(smax, &bits, sizeof(double));
+    return rc;
 }
 
 bool same_shape(const mh_grid_s* a, const mh_grid_s* b) {
@@ -886,7 +940,7 @@ int mh_stepper_advance_profiled(mh_stepper s, int nsteps, mh_stream stream, doub
 int mh_set_launch_shape(const mh_launch_shape* s) {
     if (!s) return fail(MH_ERR_INVALID, "mh_set_launch_shape: null shape");
     if (s->seg_x < 0 || s->seg_y < 0) return fail(MH_ERR_INVALID, "mh_set_launch_shape: negative segment");
-    if (s->seg_x > 0 && s->seg_x < 1) return fail(MH_ERR_INVALID, "mh_set_launch_shape: bad segment");
+    if (s->min_tiles < 0) return fail(MH_ERR_INVALID, "mh_set_launch_shape: negative min_tiles");
     if (s->y_waves != 1 && s->y_waves != 4) return fail(MH_ERR_INVALID, "mh_set_launch_shape: y_waves must be 1 or 4");
     if (s->x_layout != 0 && s->x_layout != 1) return fail(MH_ERR_INVALID, "mh_set_launch_shape: x_layout must be 0 or 1");
     if ((s->x_wpb != 1 && s->x_wpb != 4) || (s->y_wpb != 1 && s->y_wpb != 4))

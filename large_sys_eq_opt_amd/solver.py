@@ -89,6 +89,11 @@ def _stream(stream):
     return C.c_void_p(int(getattr(stream, "cuda_stream", stream)))
 
 
+def synchronize(stream=None):
+    """Waits for the work queued on a stream (the null stream by default)."""
+    call("mh_synchronize", _stream(stream))
+
+
 class Grid2DGPU:
     """Device grid: four FP64 planes (rho, rhou, rhov, E), two ghost layers."""
 
@@ -207,12 +212,12 @@ def sweep_gpu(Uin, Uout, dt, direction, solver, stream=None):
     call("mh_sweep", Uin.handle, Uout.handle, float(dt), int(direction), int(solver), _stream(stream))
 
 
-SHAPE_FIELDS = ("seg_x", "seg_y", "y_waves", "x_layout", "x_wpb", "y_wpb")
+SHAPE_FIELDS = ("seg_x", "seg_y", "y_waves", "x_layout", "x_wpb", "y_wpb", "min_tiles")
 
 
 def set_launch_shape(**kw):
     """Sets fields of the sweeps' launch shape (see mh_launch_shape):
-    seg_x, seg_y, y_waves, x_layout, x_wpb, y_wpb."""
+    seg_x, seg_y, y_waves, x_layout, x_wpb, y_wpb, min_tiles."""
     cur = get_launch_shape()
     cur.update({k: int(v) for k, v in kw.items()})
     shape = _lib.LaunchShape(*(cur[f] for f in SHAPE_FIELDS))

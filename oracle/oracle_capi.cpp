@@ -243,6 +243,26 @@ int orc_run_tangent(int nx, int ny, double dx, double dy, double* U, double* dU,
     return 0;
 }
 
+// Tangent of one sweep (frozen dt): inputs U, dU over every cell (ghosts
+// included), outputs Uout, dUout over interior cells (ghosts of Uout copy U).
+int orc_sweep_tangent(int nx, int ny, double dx, double dy, const double* U, const double* dU, double dt, int dir,
+                      int solver, double* Uout, double* dUout) {
+    const std::size_t n = cells(nx, ny);
+    std::vector<Cons<Dual>> a(n), b(n);
+    for (std::size_t k = 0; k < n; ++k)
+        for (int q = 0; q < 4; ++q) (&a[k].rho)[q] = Dual(U[4 * k + q], dU[4 * k + q]);
+    b = a;
+    Grid<Dual> gi, go;
+    for (Grid<Dual>* g : {&gi, &go}) { g->nx = nx; g->ny = ny; g->dx = dx; g->dy = dy; }
+    gi.U = a.data();
+    go.U = b.data();
+    std::vector<Cons<Dual>> ws;
+    sweep(gi, go, Dual(dt), dir, solver, ws);
+    for (std::size_t k = 0; k < n; ++k)
+        for (int q = 0; q < 4; ++q) { Uout[4 * k + q] = (&b[k].rho)[q].v; dUout[4 * k + q] = (&b[k].rho)[q].d; }
+    return 0;
+}
+
 // One x or y sweep alone (dir 0/1) over interior cells: Uout = sweep(Uin).
 int orc_sweep(int nx, int ny, double dx, double dy, const double* Uin, double* Uout, double dt, int dir, int solver) {
     Grid<double> a, b;

@@ -85,6 +85,7 @@ class Oracle:
         L.orc_run.restype = _I
         L.orc_run_tangent.argtypes = [_I, _I, _D, _D, _P, _P, _I, _I, _P, _P, _P]
         L.orc_sweep.argtypes = [_I, _I, _D, _D, _P, _P, _D, _I, _I]
+        L.orc_sweep_tangent.argtypes = [_I, _I, _D, _D, _P, _P, _D, _I, _I, _P, _P]
 
     def num_threads(self) -> int:
         return self.lib.orc_num_threads()
@@ -186,6 +187,14 @@ class Oracle:
             raise RuntimeError("compute_dt: non-positive maximum wave speed")
         return dict(U=U, steps=steps, t=t_out.value, dt=dt_log[:steps].copy(), snap_t=snap_t[:nsnap],
                     snapshots=snaps, seconds=secs.value)
+
+    def sweep_tangent(self, U, dU, dx, dy, dt, direction, solver):
+        """Tangent of one sweep: (sweep(U), d sweep(U)[dU]), dU over every cell."""
+        ny, nx = U.shape[0] - 4, U.shape[1] - 4
+        U, dU = _f64(U), _f64(dU)
+        out, dout = U.copy(), np.zeros_like(U)
+        self.lib.orc_sweep_tangent(nx, ny, dx, dy, _ptr(U), _ptr(dU), dt, direction, solver, _ptr(out), _ptr(dout))
+        return out, dout
 
     def run_tangent(self, U, dU, dx, dy, solver, dt_seq, inflow_left=None):
         ny, nx = U.shape[0] - 4, U.shape[1] - 4

@@ -70,3 +70,18 @@ def random_states(rng, n, rho=(0.1, 2.0), vel=(-1.5, 1.5), p=(0.05, 5.0)):
     V[:, 2] = rng.uniform(*vel, n)
     V[:, 3] = rng.uniform(*p, n)
     return V
+
+
+def smooth_state(oracle, nx, ny, seed=0):
+    """A smooth, everywhere non-uniform state: no limiter slope is exactly
+    zero, so the scheme is differentiable there (on piecewise-constant data
+    minmod sits at its kink and central differences see a one-sided slope)."""
+    rng = np.random.default_rng(seed)
+    y, x = (np.mgrid[0:ny + 4, 0:nx + 4] + 0.5) / np.array([ny + 4, nx + 4])[:, None, None]
+    ph = rng.uniform(0, 2 * np.pi, 6)
+    rho = 1.0 + 0.3 * np.sin(2 * np.pi * x + ph[0]) * np.cos(2 * np.pi * y + ph[1])
+    u = 0.3 * np.sin(2 * np.pi * (y + 0.37 * x) + ph[2])
+    v = 0.2 * np.cos(2 * np.pi * (x - 0.41 * y) + ph[3])
+    p = 1.0 + 0.4 * np.cos(2 * np.pi * (x + 0.83 * y) + ph[4])
+    V = np.stack([rho, u, v, p], axis=-1).reshape(-1, 4)
+    return oracle.prim_to_cons(V).reshape(ny + 4, nx + 4, 4)

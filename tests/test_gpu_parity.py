@@ -190,3 +190,49 @@ def test_shock_bubble_nd_inflow(hip, oracle, solver):
     reduced steps is used: 120 steps)."""
     o, g = _run_both(hip, oracle, "shock_bubble_adj", solver, max_steps=120)
     _assert_runs_match(o, g, solver)
+
+
+# Launch shapes: the default, short/long/whole-line segments, both LDS
+# layouts, 1 and 4 tiles per workgroup, and the small-grid segment shrink.
+LAUNCH_SHAPES = [
+    dict(seg_x=1020, seg_y=128, y_waves=1, x_layout=0, x_wpb=1, y_wpb=4, min_tiles=0),
+    dict(seg_x=60, seg_y=16, y_waves=4, x_layout=1, x_wpb=4, y_wpb=1, min_tiles=0),
+    dict(seg_x=0, seg_y=0, y_waves=1, x_layout=1, x_wpb=1, y_wpb=1, min_tiles=0),
+    dict(seg_x=100, seg_y=37, y_waves=4, x_layout=0, x_wpb=4, y_wpb=4, min_tiles=0),
+    dict(seg_x=1020, seg_y=128, y_waves=1, x_layout=0, x_wpb=1, y_wpb=4, min_tiles=2048),
+]
+
+
+@pytest.mark.parametrize("solver", [1, 2])
+@pytest.mark.parametrize("k", range(len(LAUNCH_SHAPES)))
+def test_launch_shapes_parity(hip, oracle, solver, k):
+    """Eight device-loop steps of the cylindrical blast on a ragged 300x260
+    grid under each launch shape: bitwise (HLLC) or to the exact solver's
+    tolerance, like every other run."""
+    from large_sys_eq_opt_amd import Grid2DGPU, Stepper, cases
+    from large_sys_eq_opt_amd.solver import get_launch_shape, set_launch_shape
+
+    nx, ny, steps = 300, 260, 8
+    cfg = cases.CaseConfig(nx, ny, 2, 0.0, 2.0, 0.0, 2.0, 0.8, 0.0)
+    kind, params = cases.case_params("cyl_blast")
+    U0 = oracle.init(kind, params.array(), nx, ny, cfg.bounds)
+    o = oracle.run(U0, cfg.dx, cfg.dy, solver, cfg.cfl, max_steps=steps)
+    saved = get_launch_shape()
+    try:
+        set_launch_shape(**LAUNCH_SHAPES[k])
+        g = Grid2DGPU.from_config(cfg)
+        g.upload_from_aos(U0)
+        st = Stepper(g, solver, cfg.cfl, 0.0, None, log_capacity=steps)
+        st.advance(steps)
+        st.finish()
+        _, n, dts = st.clock()
+        out = g.download_to_aos()
+    finally:
+        set_launch_shape(**saved)
+    assert n == steps
+    if BITWISE[solver]:
+        assert np.array_equal(dts, o["dt"])
+        assert np.array_equal(interior(out), interior(o["U"]))
+    else:
+        assert np.max(np.abs(dts - o["dt"]) / o["dt"]) <= EXACT_DT_TOL
+        assert rel_linf(out, o["U"]) <= EXACT_STATE_TOL

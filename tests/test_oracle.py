@@ -18,7 +18,7 @@ import os
 import numpy as np
 import pytest
 
-from conftest import interior, random_states
+from conftest import interior, random_states, smooth_state
 
 GOLD = np.load(os.path.join(os.path.dirname(__file__), "golden", "golden.npz"))
 RUN_NAMES = ["cylblast24", "sbnd48", "refblast20"]
@@ -211,21 +211,6 @@ def test_blast_mirror_symmetry(oracle):
     M = U[:, ::-1].copy()
     M[..., 1] = -M[..., 1]
     assert np.max(np.abs(U - M)) <= 1e-12 * np.max(np.abs(U))
-
-
-def smooth_state(oracle, nx, ny, seed=0):
-    """A smooth, everywhere non-uniform state: no limiter slope is exactly
-    zero, so the scheme is differentiable there (on piecewise-constant data
-    minmod sits at its kink and central differences see a one-sided slope)."""
-    rng = np.random.default_rng(seed)
-    y, x = (np.mgrid[0:ny + 4, 0:nx + 4] + 0.5) / np.array([ny + 4, nx + 4])[:, None, None]
-    ph = rng.uniform(0, 2 * np.pi, 6)
-    rho = 1.0 + 0.3 * np.sin(2 * np.pi * x + ph[0]) * np.cos(2 * np.pi * y + ph[1])
-    u = 0.3 * np.sin(2 * np.pi * (y + 0.37 * x) + ph[2])
-    v = 0.2 * np.cos(2 * np.pi * (x - 0.41 * y) + ph[3])
-    p = 1.0 + 0.4 * np.cos(2 * np.pi * (x + 0.83 * y) + ph[4])
-    V = np.stack([rho, u, v, p], axis=-1).reshape(-1, 4)
-    return oracle.prim_to_cons(V).reshape(ny + 4, nx + 4, 4)
 
 
 def test_dual_tangent_matches_central_differences(oracle):
