@@ -17,9 +17,18 @@ cpu_baseline: the reference CPU solver compiled from its sources
   (oracle/_ref) -- or the oracle port when that build is absent -- on the
   same workload at 2048^2 for a bounded number of steps, with all host
   threads OpenMP gives it.
+batched: 512 blast-wave instances (seed 0, sides log-uniform in [64,1024],
+  95.3 M cells), 500 steps with per-instance dt, one persistent launch per
+  step replayed from captured hipGraphs, schedule from the autotuner; its
+  cell-update rate and the ratio to the single-grid value.
+adjoint: the 512x256 shock-bubble adjoint case (400 HLLC steps, a
+  checkpoint every 20): the backward pass (segment recompute + adjoint
+  steps) per step over the device-resident forward step.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--size N]
-       [--solver hllc] [--no-cpu]
+       [--solver hllc] [--no-cpu] [--no-extras] [--leg batched|adjoint]
+(--leg runs one of the two extra legs alone and prints its JSON object:
+the profiling runs of those workloads use it.)
 """
 from __future__ import annotations
 
@@ -69,6 +78,70 @@ def cpu_baseline(solver: int, budget_s: float = 12.0):
             "np": np.__version__}
 
 
+def batched_leg(plain_rate: float, steps: int = 500):
+    """512-instance ensemble, graph-replayed persistent launches."""
+    from large_sys_eq_opt_amd import ensemble
+
+    spec = ensemble.ensemble_spec(512, 0)
+    b = ensemble.Batch(spec, solver=1, cfl=0.8)
+    b.initialise()
+    tune = ensemble.autotune(b, steps=4)
+    per_graph = 25
+    b.capture(per_graph)
+    b.clock()
+    t0 = time.perf_counter()
+    b.step(steps)
+    _, _, st, _ = b.clock()
+    wall = time.perf_counter() - t0
+    info = b.info()
+    b.close()
+    rate = b.cells * steps / wall
+    return {"value": rate / 1e9, "unit": "Gcell-updates/s", "instances": len(spec), "cells": b.cells,
+            "steps": steps, "ms_per_step": wall * 1e3 / steps, "ratio_vs_plain": rate / plain_rate,
+            "schedule": list(tune["schedule"]), "graph_steps": per_graph, "blocks": info["blocks"],
+            "all_instances_stepped": bool((st == steps).all())}
+
+
+def adjoint_leg():
+    """Backward-pass cost per step over the device-resident forward step on
+    the 512x256 shock-bubble adjoint case."""
+    import numpy as np
+
+    from large_sys_eq_opt_amd import adjoint as A
+    from large_sys_eq_opt_amd.solver import Grid2DGPU, Stepper, synchronize
+
+    case = A.ShockBubbleAdjoint()
+    cfg, sb, N = case.cfg, case.params, case.nsteps
+    rho_b = np.full((cfg.ny, cfg.nx), sb.bubble_rho)
+    U, _, _ = A.smooth_bubble_state(cfg, rho_b, sb.bubble_r)
+    g = Grid2DGPU.from_config(cfg)
+    best = {}
+    for _ in range(3):
+        g.upload_from_aos(U)
+        st = Stepper(g, 1, cfg.cfl, 0.0, case.bc)
+        synchronize()
+        t0 = time.perf_counter()
+        st.advance(N)
+        st.finish()
+        synchronize()
+        best["fwd"] = min(best.get("fwd", 1e9), time.perf_counter() - t0)
+        st.close()
+        case.U0.upload_from_aos(U)
+        t0 = time.perf_counter()
+        case.adj.forward(case.U0, N, cfg.cfl, None, case.UN)
+        best["ckpt"] = min(best.get("ckpt", 1e9), time.perf_counter() - t0)
+        t0 = time.perf_counter()
+        case.adj.backward(case.lamN, case.lam0)
+        best["bwd"] = min(best.get("bwd", 1e9), time.perf_counter() - t0)
+    g.close()
+    case.close()
+    f, c, b = best["fwd"] / N, best["ckpt"] / N, best["bwd"] / N
+    return {"workload": f"shock_bubble_adj {cfg.nx}x{cfg.ny} HLLC, {N} steps, checkpoint every 20",
+            "forward_us_per_step": f * 1e6, "checkpointing_forward_us_per_step": c * 1e6,
+            "backward_us_per_step": b * 1e6, "ratio_vs_forward": b / f,
+            "ratio_vs_checkpointing_forward": b / c}
+
+
 def copy_peak(nbytes: int = 4 << 30, reps: int = 10):
     """Measured device-to-device copy bandwidth (read + write bytes / s)."""
     import ctypes as C
@@ -99,6 +172,10 @@ def main():
     ap.add_argument("--size", type=int, default=8192)
     ap.add_argument("--solver", default="hllc")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extras", action="store_true", help="skip the batched and adjoint legs")
+    ap.add_argument("--leg", choices=["batched", "adjoint"], default=None,
+                    help="run only this extra leg (batched: against --plain-rate Gcell/s)")
+    ap.add_argument("--plain-rate", type=float, default=28.0)
     ap.add_argument("--copy-only", type=int, default=0,
                     help="only run the copy kernel on buffers of this many MiB (counter calibration)")
     args = ap.parse_args()
@@ -110,6 +187,12 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU only; the multi-GPU path (y-slab decomposition "
                                    "with ghost-row exchange over xGMI) is a design, not a measured result"}))
         sys.exit(2)
+    if args.leg == "batched":
+        print(json.dumps(batched_leg(args.plain_rate * 1e9)))
+        return
+    if args.leg == "adjoint":
+        print(json.dumps(adjoint_leg()))
+        return
 
     from large_sys_eq_opt_amd import Grid2DGPU, Stepper, cases, parse_riemann_solver
 
@@ -174,6 +257,9 @@ def main():
     U.close()
     out["copy_peak_GB_per_s"] = copy_peak() / 1e9
     out["roofline"]["frac_of_copy_peak"] = achieved / (out["copy_peak_GB_per_s"] * 1e9)
+    if not args.no_extras:
+        out["batched"] = batched_leg(rate)
+        out["adjoint"] = adjoint_leg()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(int(solver))
         out["cpu_baseline"].pop("np", None)
