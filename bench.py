@@ -225,7 +225,7 @@ def main():
     dom = max(("sweep_x", "sweep_y"), key=lambda k: kern[k]["avg_ms"])
     achieved = BYTES_PER_SWEEP * cells / (kern[dom]["avg_ms"] * 1e-3)
     traffic = None
-    tfile = os.path.join(ROOT, "profiles", f"traffic_{dom}_cyl_blast_{n}.json")
+    tfile = os.path.join(ROOT, "profiles", f"traffic_{dom}_{args.solver.lower()}_cyl_blast_{n}.json")
     if os.path.exists(tfile):
         with open(tfile) as f:
             traffic = json.load(f).get("bytes_per_launch")

@@ -62,13 +62,17 @@ def read_grid_csv(filename: str, nx: int, ny: int) -> np.ndarray:
 
 def run_case(case, solver, scale: int = 1, *, mode: str = "host", num_snapshots: int = 0, write_output=False,
              outdir="outputs", precision=6, order: int = 2, max_steps: int | None = None, t_end: float | None = None,
-             bc: Boundary | None = None, U0: np.ndarray | None = None, keep_snapshots=False, log=None):
-    """Runs a case through the HIP solver; returns dict(U, t, steps, dt, snap_t, seconds)."""
+             bc: Boundary | None = None, U0: np.ndarray | None = None, keep_snapshots=False, log=None,
+             cfl: float | None = None):
+    """Runs a case through the HIP solver; returns dict(U, t, steps, dt, snap_t, seconds).
+    t_end and cfl override the case's values."""
     case_id = _cases.parse_case_id(case)
     s = parse_riemann_solver(solver)
     cfg = _cases.get_n_case_config(case_id, scale)
     if t_end is not None:
         cfg.t_end = t_end
+    if cfl is not None:
+        cfg.cfl = float(cfl)
     bc = _cases.case_boundary(case_id) if bc is None else bc
     U = Grid2DGPU.from_config(cfg)
     if U0 is None:

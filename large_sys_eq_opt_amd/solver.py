@@ -200,16 +200,18 @@ def max_signal_speed_gpu(grid: Grid2DGPU, stream=None) -> float:
 
 
 def advance_second_order_gpu(Uold, Utmp, Unew, ws, dt, solver, bc: Boundary | None = None, stream=None):
-    call("mh_advance_second_order", Uold.handle, Utmp.handle, Unew.handle, float(dt), int(solver), _bc_ptr(bc),
-         _stream(stream))
+    call("mh_advance_second_order", Uold.handle, Utmp.handle, Unew.handle, float(dt),
+         int(parse_riemann_solver(solver)), _bc_ptr(bc), _stream(stream))
 
 
 def advance_first_order_gpu(Uold, Unew, dt, solver, bc: Boundary | None = None, stream=None):
-    call("mh_advance_first_order", Uold.handle, Unew.handle, float(dt), int(solver), _bc_ptr(bc), _stream(stream))
+    call("mh_advance_first_order", Uold.handle, Unew.handle, float(dt), int(parse_riemann_solver(solver)),
+         _bc_ptr(bc), _stream(stream))
 
 
 def sweep_gpu(Uin, Uout, dt, direction, solver, stream=None):
-    call("mh_sweep", Uin.handle, Uout.handle, float(dt), int(direction), int(solver), _stream(stream))
+    call("mh_sweep", Uin.handle, Uout.handle, float(dt), int(direction), int(parse_riemann_solver(solver)),
+         _stream(stream))
 
 
 SHAPE_FIELDS = ("seg_x", "seg_y", "y_waves", "x_layout", "x_wpb", "y_wpb", "min_tiles")
