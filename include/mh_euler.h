@@ -182,10 +182,12 @@ int mh_batch_grid(mh_batch b, int k, mh_grid* g);
 /* After the instances are initialised: fills ghosts, resets the clocks and
  * reduces every instance's initial max signal speed.  Synchronises. */
 int mh_batch_prepare(mh_batch b, mh_stream stream);
-/* Launch schedule: x tile row segment (0 = whole row), y tile chunk rows
- * (0 = whole column), two launches per step instead of one fused launch,
- * workgroups per CU (0 = occupancy limit).  Drops a captured graph. */
-int mh_batch_set_schedule(mh_batch b, int xseg, int ychunk, int two_launch, int blocks_per_cu);
+/* Launch schedule: x tile row segment (0 = whole row) and rows per x tile
+ * (>= 1), y tile chunk rows (0 = whole column), mode (0: one persistent
+ * launch per step; 1: two persistent launches, x then y; 2: two dispatched
+ * launches with one tile per wavefront), persistent workgroups per CU (0 =
+ * occupancy limit).  Drops a captured graph. */
+int mh_batch_set_schedule(mh_batch b, int xseg, int xrows, int ychunk, int mode, int blocks_per_cu);
 /* Enqueues n steps of every instance (graph replays where captured). */
 int mh_batch_step(mh_batch b, int nsteps, mh_stream stream);
 /* Captures steps_per_graph steps into a hipGraph used by mh_batch_step. */

@@ -23,8 +23,12 @@
 //             instance's slot with one atomic max.
 // Step parity, counters and clocks live in device memory, so the launch is
 // identical every step and a captured graph of k launches replays k steps.
-// The schedule can instead use two launches per step (x, then y): the
-// autotuner times both.
+// The schedule can instead use two launches per step (x, then y), either of
+// the same persistent kernel or of two dispatched kernels (k_batch_x,
+// k_batch_y) with one tile per wavefront: there the hardware dispatcher
+// balances the load as tiles finish, tiles are ordered largest instance
+// first, and each phase gets its own register budget.  The autotuner times
+// the modes.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -51,7 +55,7 @@ struct BatchClock {
     int step, status;
 };
 
-struct XTile { int inst, j, i0, i1; };
+struct XTile { int inst, j0, j1, i0, i1; };   // rows j0..j1-1, cells i0..i1-1
 struct YTile { int inst, cg, j0, j1; };
 
 struct Ctrl {
@@ -98,6 +102,28 @@ __device__ void grid_barrier(unsigned* bar, unsigned nblocks, int* err) {
     __syncthreads();
 }
 
+// Clock commit of phase x, shared by the persistent and dispatched kernels:
+// t += dt for every instance, clear the speed slots the y phase fills and
+// the next step's barrier counters.
+__device__ __forceinline__ void commit_clocks(const BatchArgs& a, int par) {
+    if (threadIdx.x == 0) {
+        a.ctrl->bar[par ^ 1] = 0u;
+        a.ctrl->done[par ^ 1] = 0u;
+    }
+    for (int k = threadIdx.x; k < a.ninst; k += blockDim.x) {
+        BatchClock& c = a.clk[k];
+        double dt;
+        if (inst_dt(a.inst[k], c, par, dt)) {
+            c.dt = dt;
+            c.t += dt;
+            c.step += 1;
+        } else {
+            c.status = MH_ERR_WAVESPEED;
+        }
+        c.smax[par ^ 1] = 0ull;
+    }
+}
+
 union BatchSlice {   // a wavefront's LDS: x-tile exchange or y-tile ring
     XLdsC x;
     double ring[kRing];
@@ -106,37 +132,20 @@ union BatchSlice {   // a wavefront's LDS: x-tile exchange or y-tile ring
 template <int S>
 __global__ __launch_bounds__(256) void k_batch_step(BatchArgs a) {
     __shared__ BatchSlice X[4];
-    const int w = threadIdx.x >> 6;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     const int gw = blockIdx.x * 4 + w;
     const int step = __hip_atomic_load(&a.ctrl->step, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     const int par = step & 1;
     if (a.phases & 1) {
-        if (blockIdx.x == 0) {
-            if (threadIdx.x == 0) {
-                a.ctrl->bar[par ^ 1] = 0u;
-                a.ctrl->done[par ^ 1] = 0u;
-            }
-            for (int k = threadIdx.x; k < a.ninst; k += blockDim.x) {
-                BatchClock& c = a.clk[k];
-                double dt;
-                if (inst_dt(a.inst[k], c, par, dt)) {
-                    c.dt = dt;
-                    c.t += dt;
-                    c.step += 1;
-                } else {
-                    c.status = MH_ERR_WAVESPEED;
-                }
-                c.smax[par ^ 1] = 0ull;
-            }
-        }
+        if (blockIdx.x == 0) commit_clocks(a, par);
         for (int q = a.xstart[gw]; q < a.xstart[gw + 1]; ++q) {
             const XTile t = a.xt[q];
             const BatchInst& I = a.inst[t.inst];
             double dt;
             if (!inst_dt(I, a.clk[t.inst], par, dt)) continue;
             const double dtd = dt / I.dx;
-            x_tile<S, 0>(I.U, I.T, t.j, t.i0, t.i1, dtd, 0.5 * dtd, I.bc, 1, X[w].x, lane);
+            for (int j = t.j0; j < t.j1; ++j) x_tile<S, 0>(I.U, I.T, j, t.i0, t.i1, dtd, 0.5 * dtd, I.bc, 1, X[w].x, lane);
         }
     }
     if (a.phases == 3) grid_barrier(&a.ctrl->bar[par], gridDim.x, &a.ctrl->err);
@@ -158,6 +167,55 @@ __global__ __launch_bounds__(256) void k_batch_step(BatchArgs a) {
             if (atomicAdd(&a.ctrl->done[par], 1u) == gridDim.x - 1)
                 __hip_atomic_store(&a.ctrl->step, step + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
+    }
+}
+
+// Dispatched x phase: wavefront q of the launch runs x tile q (one
+// wavefront per workgroup, as the single-grid x sweep runs best).  Block 0
+// also commits the clocks; x tiles read only smax[par], which it leaves
+// alone.
+template <int S>
+__global__ __launch_bounds__(64) void k_batch_x(BatchArgs a, int ntiles) {
+    __shared__ XLdsC X[1];
+    const int w = 0, lane = threadIdx.x;
+    const int par = __hip_atomic_load(&a.ctrl->step, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & 1;
+    if (blockIdx.x == 0) commit_clocks(a, par);
+    const int q = blockIdx.x;
+    if (q >= ntiles) return;
+    const XTile t = a.xt[q];
+    const BatchInst& I = a.inst[t.inst];
+    double dt;
+    if (!inst_dt(I, a.clk[t.inst], par, dt)) return;
+    const double dtd = dt / I.dx;
+    for (int j = t.j0; j < t.j1; ++j) x_tile<S, 0>(I.U, I.T, j, t.i0, t.i1, dtd, 0.5 * dtd, I.bc, 1, X[w], lane);
+}
+
+// Dispatched y phase: wavefront q runs y tile q; the last workgroup to
+// finish advances the step counter.
+template <int S>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 8))) void k_batch_y(BatchArgs a, int ntiles) {
+    __shared__ double ring[4][kRing];
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const int step = __hip_atomic_load(&a.ctrl->step, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const int par = step & 1;
+    const int q = blockIdx.x * 4 + w;
+    if (q < ntiles) {
+        const YTile t = a.yt[q];
+        const BatchInst& I = a.inst[t.inst];
+        double dt;
+        if (inst_dt(I, a.clk[t.inst], par, dt)) {
+            const double dtd = dt / I.dy;
+            double m = y_tile<S>(I.T, I.U, t.cg, t.j0, t.j1, dtd, 0.5 * dtd, I.bc, 1, 1, ring[w], lane);
+            m = wave_max(m);
+            if (lane == 0 && m > 0.0)
+                atomicMax(&a.clk[t.inst].smax[par ^ 1], (unsigned long long)__double_as_longlong(m));
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        if (atomicAdd(&a.ctrl->done[par], 1u) == gridDim.x - 1)
+            __hip_atomic_store(&a.ctrl->step, step + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 }
 
@@ -198,8 +256,9 @@ struct mh_batch_s {
     int* dys = nullptr;
     int nblocks = 0;
     int nxt = 0, nyt = 0;
-    // schedule
-    int xseg = 0, ychunk = 256, two_launch = 0, blocks_per_cu = 0;
+    // schedule; mode 0: one persistent launch per step, 1: two persistent
+    // launches, 2: two dispatched launches (one tile per wavefront)
+    int xseg = 0, xrows = 1, ychunk = 256, mode = 0, blocks_per_cu = 0;
     hipGraphExec_t graph = nullptr;
     hipGraph_t graph_def = nullptr;
     int graph_steps = 0;
@@ -266,14 +325,22 @@ int build_schedule(mh_batch b) {
     std::vector<XTile> xt;
     std::vector<YTile> yt;
     std::vector<double> xc, yc;
-    for (int k = 0; k < b->n; ++k) {
+    // Largest instances first: under dispatched launches the long tiles
+    // start early and the short ones fill the tail.
+    std::vector<int> order(b->n);
+    for (int k = 0; k < b->n; ++k) order[k] = k;
+    std::stable_sort(order.begin(), order.end(), [&](int p, int q) {
+        return (int64_t)b->hinst[p].U.nx * b->hinst[p].U.ny > (int64_t)b->hinst[q].U.nx * b->hinst[q].U.ny;
+    });
+    for (int k : order) {
         const int nx = b->hinst[k].U.nx, ny = b->hinst[k].U.ny;
         const int seg = b->xseg > 0 ? b->xseg : nx;
-        for (int j = 0; j < ny; ++j)
+        const int rows = std::max(1, b->xrows);
+        for (int j0 = 0; j0 < ny; j0 += rows)
             for (int i0 = 0; i0 < nx; i0 += seg) {
-                const int i1 = std::min(nx, i0 + seg);
-                xt.push_back({k, j, i0, i1});
-                xc.push_back(64.0 * ((i1 - i0 + 4 + 63) / 64));
+                const int i1 = std::min(nx, i0 + seg), j1 = std::min(ny, j0 + rows);
+                xt.push_back({k, j0, j1, i0, i1});
+                xc.push_back(64.0 * (j1 - j0) * ((i1 - i0 + 4 + 63) / 64));
             }
         const int ch = b->ychunk > 0 ? b->ychunk : ny;
         for (int cg = 0; cg * 64 < nx; ++cg)
@@ -310,9 +377,23 @@ int launch_step(mh_batch b, hipStream_t st) {
     a.ystart = b->dys;
     a.ctrl = b->dctrl;
     const dim3 g(b->nblocks), blk(256);
-    const int np = b->two_launch ? 2 : 1;
+    if (b->mode == 2) {
+        const dim3 gx((unsigned)std::max(1, b->nxt)), gy((unsigned)std::max(1, (b->nyt + 3) / 4)), bx(64);
+        switch (b->solver) {
+            case kHLL: hipLaunchKernelGGL(k_batch_x<kHLL>, gx, bx, 0, st, a, b->nxt);
+                       hipLaunchKernelGGL(k_batch_y<kHLL>, gy, blk, 0, st, a, b->nyt); break;
+            case kHLLC: hipLaunchKernelGGL(k_batch_x<kHLLC>, gx, bx, 0, st, a, b->nxt);
+                        hipLaunchKernelGGL(k_batch_y<kHLLC>, gy, blk, 0, st, a, b->nyt); break;
+            case kExact: hipLaunchKernelGGL(k_batch_x<kExact>, gx, bx, 0, st, a, b->nxt);
+                         hipLaunchKernelGGL(k_batch_y<kExact>, gy, blk, 0, st, a, b->nyt); break;
+            default: hipLaunchKernelGGL(k_batch_x<kForce>, gx, bx, 0, st, a, b->nxt);
+                     hipLaunchKernelGGL(k_batch_y<kForce>, gy, blk, 0, st, a, b->nyt); break;
+        }
+        return check_hip(hipGetLastError(), "batch step launch");
+    }
+    const int np = b->mode == 1 ? 2 : 1;
     for (int k = 0; k < np; ++k) {
-        a.phases = b->two_launch ? (k == 0 ? 1 : 2) : 3;
+        a.phases = b->mode == 1 ? (k == 0 ? 1 : 2) : 3;
         switch (b->solver) {
             case kHLL: hipLaunchKernelGGL(k_batch_step<kHLL>, g, blk, 0, st, a); break;
             case kHLLC: hipLaunchKernelGGL(k_batch_step<kHLLC>, g, blk, 0, st, a); break;
@@ -428,11 +509,13 @@ int mh_batch_prepare(mh_batch b, mh_stream stream) {
     return rc;
 }
 
-int mh_batch_set_schedule(mh_batch b, int xseg, int ychunk, int two_launch, int blocks_per_cu) {
-    if (!b || xseg < 0 || ychunk < 0 || blocks_per_cu < 0) return fail(MH_ERR_INVALID, "mh_batch_set_schedule: bad argument");
+int mh_batch_set_schedule(mh_batch b, int xseg, int xrows, int ychunk, int mode, int blocks_per_cu) {
+    if (!b || xseg < 0 || xrows < 1 || ychunk < 0 || blocks_per_cu < 0 || mode < 0 || mode > 2)
+        return fail(MH_ERR_INVALID, "mh_batch_set_schedule: bad argument");
     b->xseg = xseg;
+    b->xrows = xrows;
     b->ychunk = ychunk;
-    b->two_launch = two_launch ? 1 : 0;
+    b->mode = mode;
     b->blocks_per_cu = blocks_per_cu;
     return build_schedule(b);
 }

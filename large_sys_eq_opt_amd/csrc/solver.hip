@@ -113,7 +113,7 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(W, 8))
     if (!dt_of(a.dts, dt)) return;
     const double dtd = dt / a.d;
     const double half = 0.5 * dtd;
-    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     const int tile = blockIdx.x * WPB + w;
     if (tile >= a.ntiles) return;
     const int cg = tile % a.ngroups;
@@ -138,7 +138,7 @@ __global__ __launch_bounds__(64 * WPB) void k_sweep_x(View in, View out, SweepAr
     if (!dt_of(a.dts, dt)) return;
     const double dtd = dt / a.d;
     const double half = 0.5 * dtd;
-    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     const int tile = blockIdx.x * WPB + w;
     if (tile >= a.ntiles) return;
     const int j = tile / a.ngroups;

@@ -69,7 +69,7 @@ class Batch:
         self._h = C.c_void_p()
         call("mh_batch_create", self.n, descs, self.solver, float(cfl), None, C.byref(self._h))
         self.cells = sum(s["side"] * s["side"] for s in self.spec)
-        self.schedule = (0, 256, 0, 0)
+        self.schedule = (0, 256, 0, 0, 1)
 
     def grid(self, k: int) -> Grid2DGPU:
         """A Grid2DGPU-like view of instance k (non-owning)."""
@@ -89,9 +89,13 @@ class Batch:
             self.grid(k).init_state(2, blast_params(s))
         call("mh_batch_prepare", self._h, None)
 
-    def set_schedule(self, xseg=0, ychunk=256, two_launch=0, blocks_per_cu=0):
-        call("mh_batch_set_schedule", self._h, int(xseg), int(ychunk), int(two_launch), int(blocks_per_cu))
-        self.schedule = (int(xseg), int(ychunk), int(two_launch), int(blocks_per_cu))
+    def set_schedule(self, xseg=0, ychunk=256, mode=0, blocks_per_cu=0, xrows=1):
+        """x tiles: row segments of xseg cells (0 = whole rows), xrows rows
+        each; y tiles: chunks of ychunk rows; mode 0: one persistent launch
+        per step, 1: two persistent launches, 2: two dispatched launches with
+        one tile per wavefront."""
+        call("mh_batch_set_schedule", self._h, int(xseg), int(xrows), int(ychunk), int(mode), int(blocks_per_cu))
+        self.schedule = (int(xseg), int(ychunk), int(mode), int(blocks_per_cu), int(xrows))
 
     def capture(self, steps_per_graph: int, stream=None):
         call("mh_batch_capture", self._h, int(steps_per_graph), _stream(stream))
@@ -127,9 +131,9 @@ class Batch:
 
 
 DEFAULT_CANDIDATES = [
-    # (xseg, ychunk, two_launch, blocks_per_cu)
-    (0, 256, 0, 0), (0, 128, 0, 0), (0, 512, 0, 0), (0, 0, 0, 0),
-    (0, 256, 1, 0), (512, 256, 0, 0), (0, 256, 0, 2), (0, 128, 1, 0),
+    # (xseg, ychunk, mode, blocks_per_cu, xrows)
+    (0, 256, 0, 0, 1), (0, 128, 1, 0, 1), (0, 128, 2, 0, 1), (0, 64, 2, 0, 1), (0, 64, 2, 0, 2),
+    (0, 64, 2, 0, 4), (0, 32, 2, 0, 4), (0, 64, 2, 0, 8), (0, 128, 2, 0, 4),
 ]
 
 

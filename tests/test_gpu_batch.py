@@ -24,7 +24,10 @@ def oracle_instance(oracle, s, solver, steps):
     return oracle.run(U0, 2.0 / n, 2.0 / n, solver, 0.8, max_steps=steps)
 
 
-@pytest.mark.parametrize("schedule", [(0, 256, 0, 0), (0, 32, 1, 0), (48, 0, 0, 1)])
+# (xseg, ychunk, mode, blocks_per_cu, xrows): persistent fused / two-launch,
+# dispatched, ragged segments and chunks, several rows per x tile
+@pytest.mark.parametrize("schedule", [(0, 256, 0, 0, 1), (0, 32, 1, 0, 1), (48, 0, 0, 1, 2), (0, 64, 2, 0, 1),
+                                      (60, 16, 2, 0, 3), (0, 64, 2, 0, 8)])
 def test_batch_instances_match_oracle(hip, oracle, schedule):
     from large_sys_eq_opt_amd.ensemble import Batch
 
