@@ -18,14 +18,18 @@
 //
 // One split step U -> T -> U' (x sweep, ghosts of T, y sweep) at a given dt:
 //   T  = xsweep(U)
-//   bT = vjp_y(T, bU')  with the y-ghost adjoints of T folded by its edges
-//   bU = vjp_x(U, bT)   with the x-ghost adjoints of U folded likewise
-// (the folds happen inside k_vjp_y).  The time step is held fixed (the
-// adjoint of the step at the primal dt sequence).  mh_adjoint_forward /
-// mh_adjoint_backward run N steps with a checkpoint every K steps; the
-// backward pass recomputes each K-step segment once, keeping its states and
-// x-sweep outputs, then runs the K products in reverse: per step two sweeps,
-// two k_vjp_y and three transposes.
+//   bT = vjp_y(T, bU')  with the y-ghost adjoints of T folded by its edges,
+//                       written transposed
+//   bU = vjp_x(U, bT)   k_vjp_y on U transposed and bT (already transposed),
+//                       x-ghost adjoints folded, written back untransposed
+// (the folds and the output transposes happen inside k_vjp_y).  The time step
+// is held fixed (the adjoint of the step at the primal dt sequence).
+// mh_adjoint_forward / mh_adjoint_backward run N steps with a checkpoint
+// every K steps.  The backward pass recomputes each K-step segment once from
+// its checkpoint -- keeping its states' x-sweep outputs and transposes -- on
+// a second stream, into one of two buffers, while the products of the
+// previous segment run on the caller's stream from the other buffer; each
+// step's product is then two k_vjp_y launches.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -52,7 +56,17 @@ int max_speed_host(const mh_grid_s* g, double* smax, hipStream_t st);
 // ghost-row adjoints as they are; otherwise they are folded into rows 0 and
 // ny-1 by the edge kinds (transmissive: the ghosts copy those rows; inflow:
 // the ghosts are constants) and the ghost rows are written as zero.
-template <int S>
+// TO = 1 writes the result transposed (cell (i, c) to out(c, i), rho*u and
+// rho*v exchanged), i.e. in the layout the other direction's product reads,
+// so no separate transpose is needed; the lanes of a store then hit 64 rows
+// of one column, and consecutive rows fill those lines.
+template <int TO>
+__device__ __forceinline__ void vjp_store(const View& out, int i, int c, const Cons& v) {
+    if (TO) st(out, c, i, Cons{v.r, v.mv, v.mu, v.e});
+    else st(out, i, c, v);
+}
+
+template <int S, int TO>
 __global__ __launch_bounds__(64) void k_vjp_y(View U, View lam, View out, double dtd, double half, int seg,
                                               int nch, int ngroups, int fold, int lo_kind, int hi_kind) {
     const int lane = threadIdx.x;
@@ -108,21 +122,21 @@ __global__ __launch_bounds__(64) void k_vjp_y(View U, View lam, View out, double
             Cons bU = (c >= 0 && c < ny) ? getL((p + 5) & 3) : czero();
             c2p_vjp(getU((p + 5) & 3), a0, bU);
             if (fold < 0) {
-                if (active) st(out, i, c, bU);
+                if (active) vjp_store<TO>(out, i, c, bU);
             } else if (c < 0) {
                 if (lo_tr) acc_lo = {acc_lo.r + bU.r, acc_lo.mu + bU.mu, acc_lo.mv + bU.mv, acc_lo.e + bU.e};
-                if (active) st(out, i, c, czero());
+                if (active) vjp_store<TO>(out, i, c, czero());
             } else if (c >= ny) {
                 if (hi_tr) acc_hi = {acc_hi.r + bU.r, acc_hi.mu + bU.mu, acc_hi.mv + bU.mv, acc_hi.e + bU.e};
-                if (active) st(out, i, c, czero());
+                if (active) vjp_store<TO>(out, i, c, czero());
                 if (c == ny + 1) {
                     const Cons h = {held.r + acc_hi.r, held.mu + acc_hi.mu, held.mv + acc_hi.mv, held.e + acc_hi.e};
-                    if (active) st(out, i, ny - 1, h);
+                    if (active) vjp_store<TO>(out, i, ny - 1, h);
                 }
             } else {
                 if (c == 0) bU = {bU.r + acc_lo.r, bU.mu + acc_lo.mu, bU.mv + acc_lo.mv, bU.e + acc_lo.e};
                 if (c == ny - 1) held = bU;
-                else if (active) st(out, i, c, bU);
+                else if (active) vjp_store<TO>(out, i, c, bU);
             }
         }
         W3 = W2; W2 = W1; W1 = W0;
@@ -183,16 +197,19 @@ struct mh_adjoint_s {
     int K = 20;
     int max_steps = 0;
     mh_grid_s* T = nullptr;      // x-sweep output (single-step products)
-    mh_grid_s* bT = nullptr;     // adjoint of T
     mh_grid_s* Ut = nullptr;     // transposed state
     mh_grid_s* bTt = nullptr;    // transposed adjoint of T
-    mh_grid_s* bUt = nullptr;    // transposed adjoint of U
     mh_grid_s* work = nullptr;   // forward working state
     mh_grid_s* A0 = nullptr;     // adjoint ping-pong
     mh_grid_s* A1 = nullptr;
     std::vector<mh_grid_s*> ckpt;
-    std::vector<mh_grid_s*> segs;    // states of the segment being reversed
-    std::vector<mh_grid_s*> segT;    // their x-sweep outputs
+    // Two buffers of segment storage: the states of a segment, their
+    // x-sweep outputs and their transposes.  The backward pass recomputes
+    // segment s-1 into one buffer on `aux` while the products of segment s
+    // read the other on the caller's stream.
+    std::vector<mh_grid_s*> segs[2], segT[2], segUt[2];
+    hipStream_t aux = nullptr;
+    hipEvent_t rec_done[2] = {nullptr, nullptr}, vjp_done[2] = {nullptr, nullptr};
     std::vector<double> dts;
     int nsteps = 0;
 };
@@ -228,43 +245,65 @@ int vjp_seg(const mh_grid_desc& d) {
 }
 
 // fold < 0: ghost-row adjoints written as they are; else folded by the
-// edge kinds lo (bottom of the swept direction) and hi.
+// edge kinds lo (bottom of the swept direction) and hi.  to: write `out`
+// transposed (out has the transposed shape).
 int launch_vjp_y(const mh_grid_s* U, const mh_grid_s* lam, mh_grid_s* out, double dtd, int solver, int fold, int lo,
-                 int hi, hipStream_t st) {
+                 int hi, int to, hipStream_t st) {
     const int ngroups = (U->d.nx + 63) / 64;
     const int seg = vjp_seg(U->d);
     const int nch = std::max(1, (U->d.ny + 4) / seg);
     const dim3 g((unsigned)(ngroups * nch));
     const double half = 0.5 * dtd;
-    if (solver == kHLLC)
-        hipLaunchKernelGGL(k_vjp_y<kHLLC>, g, dim3(64), 0, st, U->view(), lam->view(), out->view(), dtd, half, seg, nch,
-                           ngroups, fold, lo, hi);
-    else
-        hipLaunchKernelGGL(k_vjp_y<kHLL>, g, dim3(64), 0, st, U->view(), lam->view(), out->view(), dtd, half, seg, nch,
-                           ngroups, fold, lo, hi);
+    const View u = U->view(), l = lam->view(), o = out->view();
+    if (solver == kHLLC) {
+        if (to) hipLaunchKernelGGL((k_vjp_y<kHLLC, 1>), g, dim3(64), 0, st, u, l, o, dtd, half, seg, nch, ngroups, fold, lo, hi);
+        else hipLaunchKernelGGL((k_vjp_y<kHLLC, 0>), g, dim3(64), 0, st, u, l, o, dtd, half, seg, nch, ngroups, fold, lo, hi);
+    } else {
+        if (to) hipLaunchKernelGGL((k_vjp_y<kHLL, 1>), g, dim3(64), 0, st, u, l, o, dtd, half, seg, nch, ngroups, fold, lo, hi);
+        else hipLaunchKernelGGL((k_vjp_y<kHLL, 0>), g, dim3(64), 0, st, u, l, o, dtd, half, seg, nch, ngroups, fold, lo, hi);
+    }
     return check_hip(hipGetLastError(), "vjp launch");
 }
 
 // lam_out = (dU'/dU)^T lam_in for one split step U -> T -> U' at time step
-// dt, given U with its x-ghost columns and T = xsweep(U) with its y-ghost
-// rows (the cells the two sweeps read).
-int step_vjp_t(mh_adjoint a, const mh_grid_s* U, const mh_grid_s* T, double dt, const mh_grid_s* lam_in,
+// dt, given Ut = U transposed (with the x-ghost columns of U) and
+// T = xsweep(U) with its y-ghost rows (the cells the two sweeps read).  The
+// y product writes the adjoint of T transposed, the x product reads it there
+// and writes lam_out back in the grid's own layout.  lam_out receives its
+// interior rows (x-ghost columns zero); its y-ghost rows are not written.
+int step_vjp_t(mh_adjoint a, const mh_grid_s* Ut, const mh_grid_s* T, double dt, const mh_grid_s* lam_in,
                mh_grid_s* lam_out, hipStream_t st) {
-    int rc = launch_vjp_y(T, lam_in, a->bT, dt / U->dy, a->solver, 1, a->bc.kind[MH_EDGE_BOTTOM],
-                          a->bc.kind[MH_EDGE_TOP], st);
-    if (!rc) rc = launch_transpose(U, a->Ut, st);
-    if (!rc) rc = launch_transpose(a->bT, a->bTt, st);
-    if (!rc) rc = launch_vjp_y(a->Ut, a->bTt, a->bUt, dt / U->dx, a->solver, 1, a->bc.kind[MH_EDGE_LEFT],
-                               a->bc.kind[MH_EDGE_RIGHT], st);
-    if (!rc) rc = launch_transpose(a->bUt, lam_out, st);
+    int rc = launch_vjp_y(T, lam_in, a->bTt, dt / T->dy, a->solver, 1, a->bc.kind[MH_EDGE_BOTTOM],
+                          a->bc.kind[MH_EDGE_TOP], 1, st);
+    if (!rc) rc = launch_vjp_y(Ut, a->bTt, lam_out, dt / T->dx, a->solver, 1, a->bc.kind[MH_EDGE_LEFT],
+                               a->bc.kind[MH_EDGE_RIGHT], 1, st);
     return rc;
 }
 
-// The same from U alone: fills U's ghosts and recomputes T.
+// The same from U alone: fills U's ghosts, recomputes T and transposes U.
 int step_vjp(mh_adjoint a, mh_grid_s* U, double dt, const mh_grid_s* lam_in, mh_grid_s* lam_out, hipStream_t st) {
     int rc = launch_fill(U, a->bc, st);
     if (!rc) rc = launch_sweep_host(U, a->T, kX, a->solver, dt, a->bc, 1, st);
-    if (!rc) rc = step_vjp_t(a, U, a->T, dt, lam_in, lam_out, st);
+    if (!rc) rc = launch_transpose(U, a->Ut, st);
+    if (!rc) rc = step_vjp_t(a, a->Ut, a->T, dt, lam_in, lam_out, st);
+    return rc;
+}
+
+// Recomputes segment s from its checkpoint (ghosts filled) into buffer b on
+// stream st: every state's x-sweep output and transpose, and the next state.
+// The sweeps' fused ghost writes give T its y-ghost rows and the next state
+// its x-ghost columns, the cells the products read.
+int recompute_segment(mh_adjoint a, int s, int b, hipStream_t st) {
+    const int k0 = s * a->K, k1 = std::min(a->nsteps, k0 + a->K);
+    int rc = launch_copy(a->ckpt[s], a->segs[b][0], st);
+    for (int n = k0; n < k1 && !rc; ++n) {
+        const int i = n - k0;
+        rc = launch_sweep_host(a->segs[b][i], a->segT[b][i], kX, a->solver, a->dts[n], a->bc, 1, st);
+        if (!rc) rc = launch_transpose(a->segs[b][i], a->segUt[b][i], st);
+        if (!rc && n + 1 < k1)
+            rc = launch_sweep_host(a->segT[b][i], a->segs[b][i + 1], kY, a->solver, a->dts[n], a->bc, 1, st);
+    }
+    if (!rc) rc = check_hip(hipEventRecord(a->rec_done[b], st), "hipEventRecord");
     return rc;
 }
 
@@ -293,10 +332,8 @@ int mh_adjoint_create(const mh_grid_desc* desc, int solver, const mh_bc* bc, int
     a->max_steps = max_steps;
     const mh_grid_desc t = transposed(*desc);
     int rc = new_grid(*desc, &a->T);
-    if (!rc) rc = new_grid(*desc, &a->bT);
     if (!rc) rc = new_grid(t, &a->Ut);
     if (!rc) rc = new_grid(t, &a->bTt);
-    if (!rc) rc = new_grid(t, &a->bUt);
     if (!rc) rc = new_grid(*desc, &a->work);
     if (!rc) rc = new_grid(*desc, &a->A0);
     if (!rc) rc = new_grid(*desc, &a->A1);
@@ -306,17 +343,27 @@ int mh_adjoint_create(const mh_grid_desc* desc, int solver, const mh_bc* bc, int
         rc = new_grid(*desc, &g);
         a->ckpt.push_back(g);
     }
-    for (int k = 0; k < (max_steps > 0 ? a->K : 0) && !rc; ++k) {
-        mh_grid_s* g = nullptr;
-        rc = new_grid(*desc, &g);
-        a->segs.push_back(g);
-        g = nullptr;
-        if (!rc) rc = new_grid(*desc, &g);
-        a->segT.push_back(g);
+    const int nseg_bufs = max_steps > a->K ? 2 : (max_steps > 0 ? 1 : 0);
+    for (int b = 0; b < nseg_bufs && !rc; ++b)
+        for (int k = 0; k < std::min(a->K, max_steps) && !rc; ++k) {
+            mh_grid_s* g = nullptr;
+            rc = new_grid(*desc, &g);
+            a->segs[b].push_back(g);
+            g = nullptr;
+            if (!rc) rc = new_grid(*desc, &g);
+            a->segT[b].push_back(g);
+            g = nullptr;
+            if (!rc) rc = new_grid(t, &g);
+            a->segUt[b].push_back(g);
+        }
+    if (!rc) rc = check_hip(hipStreamCreateWithFlags(&a->aux, hipStreamNonBlocking), "hipStreamCreate");
+    for (int b = 0; b < 2 && !rc; ++b) {
+        rc = check_hip(hipEventCreateWithFlags(&a->rec_done[b], hipEventDisableTiming), "hipEventCreate");
+        if (!rc) rc = check_hip(hipEventCreateWithFlags(&a->vjp_done[b], hipEventDisableTiming), "hipEventCreate");
     }
-    // The adjoint grids' cells that no kernel writes (ghost columns of bT and
-    // bUt) must read as zero: they reach lam_out through the transposes.
-    for (mh_grid_s* g : {a->bT, a->bTt, a->bUt, a->A0, a->A1})
+    // Cells no kernel writes (the ghost rows of the ping-pong grids) read
+    // as zero.
+    for (mh_grid_s* g : {a->bTt, a->A0, a->A1})
         if (!rc) rc = mh_grid_zero(g, nullptr);
     if (!rc) rc = check_hip(hipStreamSynchronize(nullptr), "hipStreamSynchronize");
     if (rc) { mh_adjoint_destroy(a); return rc; }
@@ -326,11 +373,18 @@ int mh_adjoint_create(const mh_grid_desc* desc, int solver, const mh_bc* bc, int
 
 int mh_adjoint_destroy(mh_adjoint a) {
     if (!a) return MH_OK;
-    for (mh_grid_s* g : {a->T, a->bT, a->Ut, a->bTt, a->bUt, a->work, a->A0, a->A1})
+    if (a->aux) (void)hipStreamSynchronize(a->aux);
+    for (mh_grid_s* g : {a->T, a->Ut, a->bTt, a->work, a->A0, a->A1})
         if (g) mh_grid_destroy(g);
     for (mh_grid_s* g : a->ckpt) if (g) mh_grid_destroy(g);
-    for (mh_grid_s* g : a->segs) if (g) mh_grid_destroy(g);
-    for (mh_grid_s* g : a->segT) if (g) mh_grid_destroy(g);
+    for (int b = 0; b < 2; ++b) {
+        for (mh_grid_s* g : a->segs[b]) if (g) mh_grid_destroy(g);
+        for (mh_grid_s* g : a->segT[b]) if (g) mh_grid_destroy(g);
+        for (mh_grid_s* g : a->segUt[b]) if (g) mh_grid_destroy(g);
+        if (a->rec_done[b]) (void)hipEventDestroy(a->rec_done[b]);
+        if (a->vjp_done[b]) (void)hipEventDestroy(a->vjp_done[b]);
+    }
+    if (a->aux) (void)hipStreamDestroy(a->aux);
     delete a;
     return MH_OK;
 }
@@ -349,16 +403,15 @@ int mh_adjoint_sweep_vjp(mh_adjoint a, mh_grid U, double dt, int dir, mh_grid la
     if (!same(U, a->d) || !same(lam_in, a->d) || !same(lam_out, a->d))
         return fail(MH_ERR_INVALID, "mh_adjoint_sweep_vjp: shape mismatch");
     hipStream_t st = (hipStream_t)stream;
+    // k_vjp_y writes only the cells the sweep reads; the rest of lam_out is zero.
+    int rc = mh_grid_zero(lam_out, stream);
     if (dir == kY) {
-        // k_vjp_y writes interior columns only; the x-ghost columns are zero.
-        int rc = mh_grid_zero(lam_out, stream);
-        if (!rc) rc = launch_vjp_y(U, lam_in, lam_out, dt / U->dy, a->solver, -1, 0, 0, st);
+        if (!rc) rc = launch_vjp_y(U, lam_in, lam_out, dt / U->dy, a->solver, -1, 0, 0, 0, st);
         return rc;
     }
-    int rc = launch_transpose(U, a->Ut, st);
+    if (!rc) rc = launch_transpose(U, a->Ut, st);
     if (!rc) rc = launch_transpose(lam_in, a->bTt, st);
-    if (!rc) rc = launch_vjp_y(a->Ut, a->bTt, a->bUt, dt / U->dx, a->solver, -1, 0, 0, st);
-    if (!rc) rc = launch_transpose(a->bUt, lam_out, st);
+    if (!rc) rc = launch_vjp_y(a->Ut, a->bTt, lam_out, dt / U->dx, a->solver, -1, 0, 0, 1, st);
     return rc;
 }
 
@@ -403,25 +456,39 @@ int mh_adjoint_backward(mh_adjoint a, mh_grid lamN, mh_grid lam0, mh_stream stre
     mh_grid_s* cur = a->A0;
     mh_grid_s* nxt = a->A1;
     const int nseg = (N + a->K - 1) / a->K;
+    const int nbuf = a->segs[1].empty() ? 1 : 2;
+    bool used[2] = {false, false};
+    // The aux stream must not start before the forward pass's checkpoints
+    // and lamN are in place: it waits for the caller's stream once.
+    if (!rc && nseg > 0) rc = check_hip(hipEventRecord(a->vjp_done[0], st), "hipEventRecord");
+    if (!rc && nseg > 0) rc = check_hip(hipStreamWaitEvent(a->aux, a->vjp_done[0], 0), "hipStreamWaitEvent");
+    if (!rc && nseg > 0) rc = recompute_segment(a, nseg - 1, (nseg - 1) % nbuf, a->aux);
     for (int s = nseg - 1; s >= 0 && !rc; --s) {
+        const int b = s % nbuf;
         const int k0 = s * a->K, k1 = std::min(N, k0 + a->K);
-        // Recompute the segment from its checkpoint (ghosts filled), keeping
-        // every state and x-sweep output.  The sweeps' fused ghost writes
-        // give T its y-ghost rows and the next state its x-ghost columns,
-        // the cells the products read.
-        rc = launch_copy(a->ckpt[s], a->segs[0], st);
-        for (int n = k0; n < k1 && !rc; ++n) {
-            rc = launch_sweep_host(a->segs[n - k0], a->segT[n - k0], kX, a->solver, a->dts[n], a->bc, 1, st);
-            if (!rc && n + 1 < k1)
-                rc = launch_sweep_host(a->segT[n - k0], a->segs[n - k0 + 1], kY, a->solver, a->dts[n], a->bc, 1, st);
+        rc = check_hip(hipStreamWaitEvent(st, a->rec_done[b], 0), "hipStreamWaitEvent");
+        if (!rc && s > 0 && nbuf == 2) {
+            // segment s-1 into the other buffer, once segment s+1's products
+            // (the buffer's last readers) are done
+            const int pb = (s - 1) % 2;
+            if (used[pb]) rc = check_hip(hipStreamWaitEvent(a->aux, a->vjp_done[pb], 0), "hipStreamWaitEvent");
+            if (!rc) rc = recompute_segment(a, s - 1, pb, a->aux);
         }
         for (int n = k1 - 1; n >= k0 && !rc; --n) {
-            rc = step_vjp_t(a, a->segs[n - k0], a->segT[n - k0], a->dts[n], cur, nxt, st);
+            rc = step_vjp_t(a, a->segUt[b][n - k0], a->segT[b][n - k0], a->dts[n], cur, nxt, st);
             std::swap(cur, nxt);
+        }
+        if (!rc) rc = check_hip(hipEventRecord(a->vjp_done[b], st), "hipEventRecord");
+        used[b] = true;
+        if (!rc && s > 0 && nbuf == 1) {
+            // one buffer: recompute the next segment after these products
+            rc = check_hip(hipStreamWaitEvent(a->aux, a->vjp_done[b], 0), "hipStreamWaitEvent");
+            if (!rc) rc = recompute_segment(a, s - 1, 0, a->aux);
         }
     }
     if (!rc) rc = launch_copy(cur, lam0, st);
     if (!rc) rc = check_hip(hipStreamSynchronize(st), "hipStreamSynchronize");
+    if (!rc) rc = check_hip(hipStreamSynchronize(a->aux), "hipStreamSynchronize");
     return rc;
 }
 
