@@ -350,6 +350,22 @@ int build_schedule(mh_batch b) {
                 yc.push_back(64.0 * (j1 - j0 + 4));
             }
     }
+    if (b->mode == 2) {
+        // Dispatched launches: longest tiles first (each instance's short
+        // remainder tiles end up in the tail, where they fill idle SIMDs).
+        auto lpt = [](auto& tiles, std::vector<double>& cost) {
+            std::vector<int> idx(tiles.size());
+            for (size_t q = 0; q < idx.size(); ++q) idx[q] = (int)q;
+            std::stable_sort(idx.begin(), idx.end(), [&](int p, int q) { return cost[p] > cost[q]; });
+            auto t2 = tiles;
+            std::vector<double> c2(cost.size());
+            for (size_t q = 0; q < idx.size(); ++q) { t2[q] = tiles[idx[q]]; c2[q] = cost[idx[q]]; }
+            tiles.swap(t2);
+            cost.swap(c2);
+        };
+        lpt(xt, xc);
+        lpt(yt, yc);
+    }
     const std::vector<int> xs = partition(xc, nw), ys = partition(yc, nw);
     free_dev(b->dxt); free_dev(b->dyt); free_dev(b->dxs); free_dev(b->dys);
     b->dxt = nullptr; b->dyt = nullptr; b->dxs = nullptr; b->dys = nullptr;

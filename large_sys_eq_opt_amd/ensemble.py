@@ -132,8 +132,8 @@ class Batch:
 
 DEFAULT_CANDIDATES = [
     # (xseg, ychunk, mode, blocks_per_cu, xrows)
-    (0, 256, 0, 0, 1), (0, 128, 1, 0, 1), (0, 128, 2, 0, 1), (0, 64, 2, 0, 1), (0, 64, 2, 0, 2),
-    (0, 64, 2, 0, 4), (0, 32, 2, 0, 4), (0, 64, 2, 0, 8), (0, 128, 2, 0, 4),
+    (0, 256, 0, 0, 1), (0, 128, 1, 0, 1), (0, 256, 2, 0, 4), (0, 192, 2, 0, 4), (0, 128, 2, 0, 4),
+    (0, 96, 2, 0, 4), (0, 64, 2, 0, 4), (0, 128, 2, 0, 1), (0, 128, 2, 0, 8),
 ]
 
 
