@@ -33,6 +33,49 @@ struct Prim { double r, u, v, p; };     // rho, u, v, p
 
 __device__ __forceinline__ bool finite(double x) { return __builtin_isfinite(x); }
 
+// Correctly rounded division and square root without the range scaling of
+// the compiler's expansions.  a / b compiles to v_div_scale (numerator and
+// denominator), v_rcp, two Newton steps, the quotient and its residual,
+// v_div_fmas and v_div_fixup; v_div_scale changes nothing unless an operand
+// is zero, denormal or near overflow, the exponents differ by 768 or more,
+// or the quotient is denormal, and v_div_fixup (kept here) settles zero,
+// infinite and NaN operands from the original operands.  sqrt(x) likewise
+// rescales only x < 2^-767 and special values.  So div_nr / sqrt_nr return
+// the same bits as a / b and sqrt(x) whenever operands and result lie well
+// inside the normal range; the sweeps keep their results (F = true) only
+// for tiles whose states lie in the fast range (range_key), where every
+// operand does.
+__device__ __forceinline__ double div_nr(double a, double b) {
+    double r = __builtin_amdgcn_rcp(b);
+    double e = __builtin_fma(-b, r, 1.0);
+    r = __builtin_fma(r, e, r);
+    e = __builtin_fma(-b, r, 1.0);
+    r = __builtin_fma(r, e, r);
+    double q = a * r;
+    const double t = __builtin_fma(-b, q, a);
+    q = __builtin_fma(t, r, q);
+    return __builtin_amdgcn_div_fixup(q, b, a);
+}
+__device__ __forceinline__ double sqrt_nr(double x) {
+    const double y = __builtin_amdgcn_rsq(x);
+    double g = x * y, h = y * 0.5;
+    const double r = __builtin_fma(-h, g, 0.5);
+    g = __builtin_fma(g, r, g);
+    h = __builtin_fma(h, r, h);
+    double d = __builtin_fma(-g, g, x);
+    g = __builtin_fma(d, h, g);
+    d = __builtin_fma(-g, g, x);
+    return __builtin_fma(d, h, g);
+}
+template <bool F> __device__ __forceinline__ double dv(double a, double b) {
+    if constexpr (F) return div_nr(a, b);
+    else return a / b;
+}
+template <bool F> __device__ __forceinline__ double sq(double x) {
+    if constexpr (F) return sqrt_nr(x);
+    else return sqrt(x);
+}
+
 // libm fmin/fmax and std::min/max -- used exactly where the reference uses
 // each.  Every fmin/fmax operand on this path is finite and non-zero ties
 // have equal bits (the operands are sums with a positive sound speed, or the
@@ -43,8 +86,9 @@ __device__ __forceinline__ double smin(double a, double b) { return (b < a) ? b 
 __device__ __forceinline__ double smax(double a, double b) { return (a < b) ? b : a; }
 
 // head/physics.hpp:13-20
+template <bool F = false>
 __device__ __forceinline__ Prim c2p(const Cons& U) {
-    const double inv_rho = 1.0 / U.r;
+    const double inv_rho = dv<F>(1.0, U.r);
     const double u = U.mu * inv_rho;
     const double v = U.mv * inv_rho;
     const double kinetic = 0.5 * U.r * (u * u + v * v);
@@ -52,12 +96,14 @@ __device__ __forceinline__ Prim c2p(const Cons& U) {
 }
 
 // head/physics.hpp:22-28
+template <bool F = false>
 __device__ __forceinline__ Cons p2c(const Prim& V) {
     const double kinetic = 0.5 * V.r * (V.u * V.u + V.v * V.v);
-    return {V.r, V.r * V.u, V.r * V.v, V.p / kGm1 + kinetic};
+    return {V.r, V.r * V.u, V.r * V.v, dv<F>(V.p, kGm1) + kinetic};
 }
 
-__device__ __forceinline__ double sound(const Prim& V) { return sqrt(kGamma * V.p / V.r); }  // :39-41
+template <bool F = false>
+__device__ __forceinline__ double sound(const Prim& V) { return sq<F>(dv<F>(kGamma * V.p, V.r)); }  // :39-41
 
 // Physical flux of U along D given V = c2p(U) (head/physics.hpp:56-74).
 template <int D> __device__ __forceinline__ Cons pflux(const Cons& U, const Prim& V) {
@@ -83,12 +129,12 @@ __device__ __forceinline__ Cons avg(const Cons& a, const Cons& b) {  // 0.5 * (a
 // ---------------------------------------------------------------------------
 // HLL (head/riemann.hpp:87-129); VL/VR = c2p(UL/UR).
 // ---------------------------------------------------------------------------
-template <int D>
+template <int D, bool F = false>
 __device__ __forceinline__ Cons hll(const Cons& UL, const Prim& VL, const Cons& UR, const Prim& VR) {
     const Cons FL = pflux<D>(UL, VL);
     const Cons FR = pflux<D>(UR, VR);
     if (!phys_riemann(VL) || !phys_riemann(VR)) return avg(FL, FR);
-    const double aL = sound(VL), aR = sound(VR);
+    const double aL = sound<F>(VL), aR = sound<F>(VR);
     const double unL = un_of<D>(VL), unR = un_of<D>(VR);
     const double SL = lmin(unL - aL, unR - aR);
     const double SR = lmax(unL + aL, unR + aR);
@@ -97,34 +143,34 @@ __device__ __forceinline__ Cons hll(const Cons& UL, const Prim& VL, const Cons& 
     const double denom = SR - SL;
     if (fabs(denom) < 1.0e-14) return avg(FL, FR);
     const double sls = SL * SR;
-    return {(SR * FL.r - SL * FR.r + sls * (UR.r - UL.r)) / denom,
-            (SR * FL.mu - SL * FR.mu + sls * (UR.mu - UL.mu)) / denom,
-            (SR * FL.mv - SL * FR.mv + sls * (UR.mv - UL.mv)) / denom,
-            (SR * FL.e - SL * FR.e + sls * (UR.e - UL.e)) / denom};
+    return {dv<F>(SR * FL.r - SL * FR.r + sls * (UR.r - UL.r), denom),
+            dv<F>(SR * FL.mu - SL * FR.mu + sls * (UR.mu - UL.mu), denom),
+            dv<F>(SR * FL.mv - SL * FR.mv + sls * (UR.mv - UL.mv), denom),
+            dv<F>(SR * FL.e - SL * FR.e + sls * (UR.e - UL.e), denom)};
 }
 
 // ---------------------------------------------------------------------------
 // HLLC (head/riemann.hpp:135-271)
 // ---------------------------------------------------------------------------
-template <int D>
+template <int D, bool F = false>
 __device__ __forceinline__ Cons hllc_star(const Cons& U, const Prim& V, double S, double Sstar) {
     const double rho = V.r, p = V.p;
     const double un = un_of<D>(V), ut = ut_of<D>(V);
     const double denom = S - Sstar;
     if (fabs(denom) < 1.0e-14) return U;
-    const double rho_star = rho * (S - un) / denom;
+    const double rho_star = dv<F>(rho * (S - un), denom);
     if (!finite(rho_star) || rho_star <= 0.0) return U;
-    const double E_per_rho = U.e / rho;
-    const double E_star = rho_star * (E_per_rho + (Sstar - un) * (Sstar + p / (rho * (S - un))));
+    const double E_per_rho = dv<F>(U.e, rho);
+    const double E_star = rho_star * (E_per_rho + (Sstar - un) * (Sstar + dv<F>(p, rho * (S - un))));
     if (!finite(E_star)) return U;
     if (D == kX) return {rho_star, rho_star * Sstar, rho_star * ut, E_star};
     return {rho_star, rho_star * ut, rho_star * Sstar, E_star};
 }
 
-template <int D>
+template <int D, bool F = false>
 __device__ __forceinline__ Cons hllc(const Cons& UL, const Prim& VL, const Cons& UR, const Prim& VR) {
-    if (!phys_riemann(VL) || !phys_riemann(VR)) return hll<D>(UL, VL, UR, VR);
-    const double aL = sound(VL), aR = sound(VR);
+    if (!phys_riemann(VL) || !phys_riemann(VR)) return hll<D, F>(UL, VL, UR, VR);
+    const double aL = sound<F>(VL), aR = sound<F>(VR);
     const double unL = un_of<D>(VL), unR = un_of<D>(VR);
     const double pL = VL.p, pR = VR.p, rhoL = VL.r, rhoR = VR.r;
     const double SL = lmin(unL - aL, unR - aR);
@@ -135,17 +181,17 @@ __device__ __forceinline__ Cons hllc(const Cons& UL, const Prim& VL, const Cons&
     if (SR <= 0.0) return FR;
     const double numerator = pR - pL + rhoL * unL * (SL - unL) - rhoR * unR * (SR - unR);
     const double denominator = rhoL * (SL - unL) - rhoR * (SR - unR);
-    if (fabs(denominator) < 1.0e-14) return hll<D>(UL, VL, UR, VR);
-    const double Sstar = numerator / denominator;
-    if (!finite(Sstar)) return hll<D>(UL, VL, UR, VR);
+    if (fabs(denominator) < 1.0e-14) return hll<D, F>(UL, VL, UR, VR);
+    const double Sstar = dv<F>(numerator, denominator);
+    if (!finite(Sstar)) return hll<D, F>(UL, VL, UR, VR);
     // One star state is formed, on the side the sign of S* picks
     // (neighbouring faces mostly agree, so the branch rarely diverges).
     if (Sstar >= 0.0) {
-        const Cons Us = hllc_star<D>(UL, VL, SL, Sstar);
+        const Cons Us = hllc_star<D, F>(UL, VL, SL, Sstar);
         return {FL.r + SL * (Us.r - UL.r), FL.mu + SL * (Us.mu - UL.mu), FL.mv + SL * (Us.mv - UL.mv),
                 FL.e + SL * (Us.e - UL.e)};
     }
-    const Cons Us = hllc_star<D>(UR, VR, SR, Sstar);
+    const Cons Us = hllc_star<D, F>(UR, VR, SR, Sstar);
     return {FR.r + SR * (Us.r - UR.r), FR.mu + SR * (Us.mu - UR.mu), FR.mv + SR * (Us.mv - UR.mv),
             FR.e + SR * (Us.e - UR.e)};
 }
@@ -305,10 +351,12 @@ __device__ __forceinline__ Cons exact(const Cons& UL, const Prim& VL, const Cons
     return pflux<D>(Us, Vs);
 }
 
-template <int S, int D>
+// F selects the unscaled division / square root for HLL and HLLC (FORCE and
+// the exact solver keep the compiler's expansions).
+template <int S, int D, bool F = false>
 __device__ __forceinline__ Cons riemann(const Cons& UL, const Prim& VL, const Cons& UR, const Prim& VR, int& iters) {
-    if (S == kHLL) return hll<D>(UL, VL, UR, VR);
-    if (S == kHLLC) return hllc<D>(UL, VL, UR, VR);
+    if (S == kHLL) return hll<D, F>(UL, VL, UR, VR);
+    if (S == kHLLC) return hllc<D, F>(UL, VL, UR, VR);
     if (S == kForce) return force<D>(UL, VL, UR, VR);
     return exact<D>(UL, VL, UR, VR, iters);
 }
@@ -331,7 +379,7 @@ __device__ __forceinline__ double minmod(double a, double b) {
 
 struct Edge { Cons U; Prim V; };
 
-template <int D>
+template <int D, bool F = false>
 __device__ __forceinline__ void reconstruct(const Prim& Wm, const Prim& Wc, const Prim& Wp, double half, Edge& L,
                                             Edge& R) {
     const Prim s = {minmod(Wc.r - Wm.r, Wp.r - Wc.r), minmod(Wc.u - Wm.u, Wp.u - Wc.u),
@@ -346,19 +394,51 @@ __device__ __forceinline__ void reconstruct(const Prim& Wm, const Prim& Wc, cons
         if (!okL) WL = Wc;
         if (!okR) WR = Wc;
     }
-    const Cons UL = p2c(WL), UR = p2c(WR);
-    const Prim VL = c2p(UL), VR = c2p(UR);
+    const Cons UL = p2c<F>(WL), UR = p2c<F>(WR);
+    const Prim VL = c2p<F>(UL), VR = c2p<F>(UR);
     const Cons FL = pflux<D>(UL, VL), FR = pflux<D>(UR, VR);
     const Cons h = {half * (FR.r - FL.r), half * (FR.mu - FL.mu), half * (FR.mv - FL.mv), half * (FR.e - FL.e)};
     L.U = {UL.r - h.r, UL.mu - h.mu, UL.mv - h.mv, UL.e - h.e};
     R.U = {UR.r - h.r, UR.mu - h.mu, UR.mv - h.mv, UR.e - h.e};
-    L.V = c2p(L.U);
-    R.V = c2p(R.U);
+    L.V = c2p<F>(L.U);
+    R.V = c2p<F>(R.U);
     const bool eL = phys_recon(L.V), eR = phys_recon(R.V);
     if (!__all(eL && eR)) {
         if (!eL) { L.U = UL; L.V = VL; }
         if (!eR) { R.U = UR; R.V = VR; }
     }
+}
+
+// Tiles whose states all lie in the fast range, swept at a dt/d in
+// [2^-20, 2^20] (dt_in_fast_range), may use div_nr / sqrt_nr in the
+// reconstruction, HLL/HLLC and the signal speed: bit-identical to the
+// compiler's division and square root there.  The fast range: rho, p and E
+// in [2^-20, 2^20).  Then E >= (mu^2 + mv^2) / (2 rho) (p > 0) bounds
+// |rho*u|, |rho*v| and |u|, |v| by 2^20.5, the reconstructed states stay
+// within a factor 2 of these, their fluxes below 2^90, the evolved edge
+// states below 2^110 (nonzero densities above 2^-73: a difference of
+// in-range values is a multiple of their last place), and an evolved state
+// that passes the physicality check (rho, p > 1e-12) has |u| <= 2^150.
+// Every division then has a nonzero numerator above 2^-300 (or zero,
+// settled by v_div_fixup), numerator/denominator exponents less than 500
+// apart and a normal quotient, and every square root an argument above
+// 2^-310 -- none of the cases in which v_div_scale or the square root's
+// scaling changes anything (exponent gap >= 768, denormals, numerator below
+// 2^-970, argument below 2^-767).
+//
+// The test works on the high words (sign, exponent, top of the mantissa):
+// for x >= 0 the order of the high words is that of the values, so x lies
+// in [2^-20, 2^20) exactly when hi(x) - hi(2^-20) < hi(2^20) - hi(2^-20) as
+// unsigned integers; negative values, infinities and NaN fall outside.  A
+// tile keeps the running maximum of range_key over the states it reads (an
+// integer per lane rather than a lane mask: the y sweep has no scalar
+// registers to spare) and compares it with kRangeKeyEnd once.
+__device__ __forceinline__ unsigned hi_word(double x) { return (unsigned)(__double_as_longlong(x) >> 32); }
+constexpr unsigned kRangeKeyBase = 0x3EB00000u;                   // hi_word(2^-20)
+constexpr unsigned kRangeKeyEnd = 0x41300000u - kRangeKeyBase;    // hi_word(2^20) - base
+__device__ __forceinline__ unsigned range_key(double rho, double e, double p) {
+    const unsigned a = hi_word(rho) - kRangeKeyBase, b = hi_word(e) - kRangeKeyBase, c = hi_word(p) - kRangeKeyBase;
+    return max(a, max(b, c));
 }
 
 // Cell update U - (dt/d) * (Fp - Fm)  (src/cpu/solver_cpu.cpp:452-453).
@@ -368,9 +448,10 @@ __device__ __forceinline__ Cons update(const Cons& U, double dtd, const Cons& Fm
 }
 
 // max(|u|+a, |v|+a) as src/cpu/solver_cpu.cpp:295-300 forms it.
+template <bool F = false>
 __device__ __forceinline__ double signal_speed(const Cons& U) {
-    const Prim V = c2p(U);
-    const double a = sound(V);
+    const Prim V = c2p<F>(U);
+    const double a = sound<F>(V);
     return smax(fabs(V.u) + a, fabs(V.v) + a);
 }
 

@@ -38,6 +38,11 @@ struct mh_grid_s {
     int64_t rp, ps;      // row pitch, plane stride (doubles)
     double* base;        // allocation start
     unsigned long long* aux;  // 8 device words of per-grid scratch (max-speed slot), or null
+    // y-sweep redo list of tiles whose states left the fast-path range:
+    // redo[0] count, redo[1] workgroups done, redo[2..] tile indices (null
+    // for the grids of a batch)
+    int* redo = nullptr;
+    int64_t redo_cap = 0;
     int device;
     int owns;            // 1: base/aux were allocated for this grid (0: a view into a batch arena)
     mh::View view() const {

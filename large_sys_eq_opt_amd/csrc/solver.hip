@@ -99,6 +99,7 @@ struct SweepArgs {
     // goes -- a fixed slot, or (clock_speed) the clock's inactive slot.
     unsigned long long* speed;
     int clock_speed;
+    int* redo;         // y sweep: the output grid's redo list (see k_sweep_y)
 };
 
 // Sweep kernels: WPB independent wavefronts per workgroup, one tile each
@@ -106,7 +107,18 @@ struct SweepArgs {
 // wavefronts per SIMD the register allocation aims for (1 = compiler's
 // choice; four fit the HLL/HLLC/FORCE y tile in 128 VGPRs).  For the x
 // sweep, L selects the LDS exchange layout.
-template <int S, int W, int WPB>
+//
+// The y sweep runs each tile's fast pass (y_tile_pass<S, true>).  A tile
+// whose states leave the fast-path range is, with RL = 1, appended to the
+// output grid's redo list and left to k_sweep_y_redo, which the host
+// launches right after on the same stream (it adds the tile's speed
+// maximum); with RL = 0 the IEEE pass runs in place.  Keeping the IEEE pass
+// out of this kernel keeps its hot loop lean.
+__device__ __forceinline__ unsigned long long* speed_slot(const SweepArgs& a) {
+    return a.clock_speed ? &a.dts.clock->smax[a.dts.clock->slot ^ 1] : a.speed;
+}
+
+template <int S, int W, int WPB, int RL>
 __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(W, 8))) void k_sweep_y(View in, View out,
                                                                                                SweepArgs a) {
     double dt;
@@ -122,13 +134,51 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(W, 8))
     const int j1 = min(j0 + a.seg, in.ny);
     const int want = (a.speed || a.clock_speed) ? 1 : 0;
     __shared__ double ring[WPB][kRing];
-    double m = y_tile<S>(in, out, cg, j0, j1, dtd, half, a.bc, a.ghosts, want, ring[w], lane);
+    double m = 0.0;
+    if (!(dt_in_fast_range(dtd) &&
+          y_tile_pass<S, true>(in, out, cg, j0, j1, dtd, half, a.bc, a.ghosts, want, ring[w], lane, m))) {
+        if constexpr (RL) {
+            if (lane == 0) a.redo[2 + atomicAdd(a.redo, 1)] = tile;
+            return;
+        } else {
+            y_tile_pass<S, false>(in, out, cg, j0, j1, dtd, half, a.bc, a.ghosts, want, ring[w], lane, m);
+        }
+    }
     if (want) {
         m = wave_max(m);
-        if (lane == 0) {
-            unsigned long long* slot = a.clock_speed ? &a.dts.clock->smax[a.dts.clock->slot ^ 1] : a.speed;
-            atomic_max_pos(slot, m);
+        if (lane == 0) atomic_max_pos(speed_slot(a), m);
+    }
+}
+
+// IEEE pass over the tiles the preceding k_sweep_y put on the redo list;
+// the last workgroup to finish empties the list for the next launch.
+template <int S>
+__global__ __launch_bounds__(256) void k_sweep_y_redo(View in, View out, SweepArgs a) {
+    __shared__ double ring[4][kRing];
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const int n = __hip_atomic_load(a.redo, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    double dt;
+    if (n > 0 && dt_of(a.dts, dt)) {
+        const double dtd = dt / a.d;
+        const double half = 0.5 * dtd;
+        const int want = (a.speed || a.clock_speed) ? 1 : 0;
+        for (int q = blockIdx.x * 4 + w; q < n; q += gridDim.x * 4) {
+            const int tile = a.redo[2 + q];
+            const int cg = tile % a.ngroups;
+            const int j0 = (tile / a.ngroups) * a.seg;
+            const int j1 = min(j0 + a.seg, in.ny);
+            double m = 0.0;
+            y_tile_pass<S, false>(in, out, cg, j0, j1, dtd, half, a.bc, a.ghosts, want, ring[w], lane, m);
+            if (want) {
+                m = wave_max(m);
+                if (lane == 0) atomic_max_pos(speed_slot(a), m);
+            }
         }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0 && atomicAdd(a.redo + 1, 1) == (int)gridDim.x - 1) {
+        a.redo[0] = 0;
+        a.redo[1] = 0;
     }
 }
 
@@ -365,7 +415,12 @@ void go_x(dim3 g, hipStream_t st, const View& vi, const View& vo, const SweepArg
 }
 template <int S, int W, int WPB>
 void go_y(dim3 g, hipStream_t st, const View& vi, const View& vo, const SweepArgs& a) {
-    hipLaunchKernelGGL((k_sweep_y<S, W, WPB>), g, dim3(64 * WPB), 0, st, vi, vo, a);
+    if (a.redo) {
+        hipLaunchKernelGGL((k_sweep_y<S, W, WPB, 1>), g, dim3(64 * WPB), 0, st, vi, vo, a);
+        hipLaunchKernelGGL(k_sweep_y_redo<S>, dim3(64), dim3(256), 0, st, vi, vo, a);
+    } else {
+        hipLaunchKernelGGL((k_sweep_y<S, W, WPB, 0>), g, dim3(64 * WPB), 0, st, vi, vo, a);
+    }
 }
 template <int S>
 void go_x_s(int layout, int wpb, dim3 g, hipStream_t st, const View& vi, const View& vo, const SweepArgs& a) {
@@ -414,6 +469,7 @@ int launch_sweep(const mh_grid_s* in, const mh_grid_s* out, int dir, int solver,
     a.ghosts = ghosts;
     a.speed = speed;
     a.clock_speed = clock_speed;
+    a.redo = nullptr;
     const View vi = in->view(), vo = out->view();
     if (dir == kX) {
         a.d = in->dx;
@@ -434,6 +490,7 @@ int launch_sweep(const mh_grid_s* in, const mh_grid_s* out, int dir, int solver,
         a.seg = seg_y_for(in);
         a.ngroups = (in->d.nx + 63) / 64;
         a.ntiles = a.ngroups * ((in->d.ny + a.seg - 1) / a.seg);
+        if (out->redo && a.ntiles <= out->redo_cap) a.redo = out->redo;
         const int wpb = a.ntiles < g_shape.min_tiles ? 1 : g_shape.y_wpb;
         const dim3 g((unsigned)((a.ntiles + wpb - 1) / wpb));
         const int W = g_shape.y_waves;
@@ -528,6 +585,17 @@ int mh_grid_create(const mh_grid_desc* desc, mh_grid* out) {
         delete g;
         return check_hip(e2, "hipMalloc(grid aux)");
     }
+    // y-sweep redo list: room for every tile of the finest chunking (1 row)
+    g->redo_cap = (int64_t)((desc->nx + 63) / 64) * desc->ny;
+    hipError_t e3 = hipMalloc((void**)&g->redo, sizeof(int) * (size_t)(2 + g->redo_cap));
+    if (e3 == hipSuccess) e3 = hipMemset(g->redo, 0, 2 * sizeof(int));
+    if (e3 != hipSuccess) {
+        (void)hipFree(g->base);
+        (void)hipFree(g->aux);
+        if (g->redo) (void)hipFree(g->redo);
+        delete g;
+        return check_hip(e3, "hipMalloc(grid redo list)");
+    }
     *out = g;
     return MH_OK;
 }
@@ -537,7 +605,9 @@ int mh_grid_destroy(mh_grid g) {
     if (!g->owns) return fail(MH_ERR_INVALID, "mh_grid_destroy: grid belongs to a batch");
     hipError_t e = hipFree(g->base);
     const hipError_t e2 = hipFree(g->aux);
+    const hipError_t e3 = g->redo ? hipFree(g->redo) : hipSuccess;
     if (e == hipSuccess) e = e2;
+    if (e == hipSuccess) e = e3;
     delete g;
     return check_hip(e, "hipFree(grid)");
 }

@@ -82,17 +82,31 @@ struct XLdsC {
     double cF[4];
 };
 
-constexpr int kRing = 768;   // y tile: three rows of four planes, 64 lanes
+// y tile: three rows of four planes, 64 lanes (768 doubles), then each
+// lane's range key (low word of slot 768 + lane)
+constexpr int kRing = 832;
 
-// y tile over columns [64*cg, 64*cg+64) and rows [j0, j1).  Returns the
-// lane's running max of the output's signal speed when `speed` is set
-// (std::max semantics: a NaN speed never enters), 0 otherwise.  The
-// conserved states of rows p-1 and p-2 wait for the update in a three-row
-// LDS ring (each lane touches only its own column, so program order is the
-// only ordering needed); `ring` is the wavefront's kRing-double LDS slice.
-template <int S>
-__device__ __forceinline__ double y_tile(const View& in, const View& out, int cg, int j0, int j1, double dtd,
-                                         double half, const Bc& bc, int ghosts, int speed, double* ring, int lane) {
+// Fast-path selection shared by the tiles: a tile's results computed with
+// div_nr / sqrt_nr equal the IEEE ones when every state the tile reads lies
+// in the fast range (range_key < kRangeKeyEnd) and dt/d is within
+// [2^-20, 2^20] (euler_device.h gives the bound).  dt/d is tested before the
+// tile; the states' keys are accumulated as they are read and voted on once,
+// after the tile.
+__device__ __forceinline__ bool dt_in_fast_range(double dtd) { return dtd >= 0x1p-20 && dtd <= 0x1p20; }
+
+// One pass of a y tile with the fast (F = true) or IEEE (F = false)
+// division and square root.  The fast pass runs the whole tile
+// speculatively: each row's range key is folded into a lane-local maximum
+// (no vote, no exit inside the loop) and one wavefront vote at the end says
+// whether the tile's results stand (returns false if not).  Then the IEEE
+// pass must redo the tile: it reads only the input grid and rewrites every
+// cell and ghost the fast pass stored, and the fast pass's speed maximum is
+// dropped.  The IEEE pass stays out of the fast pass's loop (and, for the
+// single-grid sweep, out of its kernel), which keeps the hot loop lean.
+template <int S, bool F>
+__device__ __forceinline__ bool y_tile_pass(const View& in, const View& out, int cg, int j0, int j1, double dtd,
+                                            double half, const Bc& bc, int ghosts, int speed, double* ring, int lane,
+                                            double& m) {
     const int i = cg * 64 + lane;
     const bool active = i < in.nx;
     const int ic = active ? i : in.nx - 1;
@@ -105,49 +119,64 @@ __device__ __forceinline__ double y_tile(const View& in, const View& out, int cg
         const double* q = ring + slot * 256 + lane;
         return Cons{q[0], q[64], q[128], q[192]};
     };
+    // converts row state U to W, folding its range key into the lane's key
+    // slot -- in LDS: one more register would take the tile past 128 VGPRs
+    unsigned* key = reinterpret_cast<unsigned*>(ring + 768 + lane);
+    if constexpr (F) __hip_atomic_store(key, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+    auto conv = [&](const Cons& U, Prim& W) {
+        W = c2p<F>(U);
+        if constexpr (F)
+            __hip_atomic_fetch_max(key, range_key(U.r, U.e, W.p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+    };
+    m = 0.0;
     // rows j0-2 and j0-1: convert only
     Cons U0 = ld(in, ic, j0 - 2);
-    Prim W2 = c2p(U0);
+    Prim W2, W1, W0;
+    conv(U0, W2);
     put(0, U0);
     U0 = ld(in, ic, j0 - 1);
-    Prim W1 = c2p(U0);
+    conv(U0, W1);
     put(1, U0);
     Cons Un = ld(in, ic, j0);
     Edge L, R, Rp;
     {   // row p = j0: reconstruct j0-1
         U0 = Un;
         Un = ld(in, ic, j0 + 1);
-        const Prim W0 = c2p(U0);
+        conv(U0, W0);
         put(2, U0);
-        reconstruct<kY>(W2, W1, W0, half, L, Rp);
+        reconstruct<kY, F>(W2, W1, W0, half, L, Rp);
         W2 = W1; W1 = W0;
     }
     Cons Fp;
     {   // row p = j0+1: reconstruct j0, face j0-1/2
         U0 = Un;
         Un = ld(in, ic, j0 + 2);
-        const Prim W0 = c2p(U0);
+        conv(U0, W0);
         put(0, U0);   // row j0-2 is no longer needed
-        reconstruct<kY>(W2, W1, W0, half, L, R);
-        Fp = riemann<S, kY>(Rp.U, Rp.V, L.U, L.V, it);
+        reconstruct<kY, F>(W2, W1, W0, half, L, R);
+        Fp = riemann<S, kY, F>(Rp.U, Rp.V, L.U, L.V, it);
         Rp = R;
         W2 = W1; W1 = W0;
     }
-    double m = 0.0;
     // rows p = j0+2 .. j1+1; the ring slot of row p is (p - j0 + 2) % 3
     int slot_new = 1, slot_old = 2;   // slots of rows p and p-2
     for (int p = j0 + 2; p <= j1 + 1; ++p) {
         U0 = Un;
         Un = ld(in, ic, min(p + 1, j1 + 1));
-        const Prim W0 = c2p(U0);
-        reconstruct<kY>(W2, W1, W0, half, L, R);
-        const Cons F = riemann<S, kY>(Rp.U, Rp.V, L.U, L.V, it);
+        conv(U0, W0);
+        reconstruct<kY, F>(W2, W1, W0, half, L, R);
+        const Cons Fc = riemann<S, kY, F>(Rp.U, Rp.V, L.U, L.V, it);
         const int r = p - 2;
-        const Cons Unew = update(get(slot_old), dtd, Fp, F);
+        const Cons Unew = update(get(slot_old), dtd, Fp, Fc);
         put(slot_new, U0);
         if (active) {
             st(out, i, r, Unew);
-            if (speed) m = smax(m, signal_speed(Unew));
+            // From rows in range, the new state's rho and p are zero or within
+            // about 2^-150..2^400 (differences of in-range values cannot fall
+            // below their unit in the last place), so its signal speed's
+            // 1/rho, gamma*p/rho and square root stay exact with the fast
+            // sequences; a NaN speed never enters the maximum.
+            if (speed) m = smax(m, signal_speed<F>(Unew));
             if (ghosts) {
                 if (i == 0) {
                     const Cons G = bc.kind[MH_EDGE_LEFT] == MH_BC_INFLOW ? bc.s[MH_EDGE_LEFT] : Unew;
@@ -161,12 +190,32 @@ __device__ __forceinline__ double y_tile(const View& in, const View& out, int cg
                 }
             }
         }
-        Fp = F;
+        Fp = Fc;
         Rp = R;
         W2 = W1; W1 = W0;
         slot_old = slot_old == 2 ? 0 : slot_old + 1;
         slot_new = slot_new == 2 ? 0 : slot_new + 1;
     }
+    if constexpr (F)
+        return !__any(__hip_atomic_load(key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT) >= kRangeKeyEnd);
+    return true;
+}
+
+// y tile over columns [64*cg, 64*cg+64) and rows [j0, j1).  Returns the
+// lane's running max of the output's signal speed when `speed` is set
+// (std::max semantics: a NaN speed never enters), 0 otherwise.  Each lane
+// marches up its column; the conserved states of rows p-1 and p-2 wait for
+// the update in a three-row LDS ring (each lane touches only its own
+// column, so program order is the only ordering needed); `ring` is the
+// wavefront's kRing-double LDS slice.  The fast pass runs first when dt/d is
+// in range (see y_tile_pass).
+template <int S>
+__device__ __forceinline__ double y_tile(const View& in, const View& out, int cg, int j0, int j1, double dtd,
+                                         double half, const Bc& bc, int ghosts, int speed, double* ring, int lane) {
+    double m = 0.0;
+    if (dt_in_fast_range(dtd) && y_tile_pass<S, true>(in, out, cg, j0, j1, dtd, half, bc, ghosts, speed, ring, lane, m))
+        return m;
+    y_tile_pass<S, false>(in, out, cg, j0, j1, dtd, half, bc, ghosts, speed, ring, lane, m);
     return m;
 }
 
@@ -175,17 +224,19 @@ __device__ __forceinline__ double y_tile(const View& in, const View& out, int cg
 // W and U of cells p-1, p-2 and the right edge state and flux of the left
 // neighbour come from lanes l-1, l-2 through LDS (the carry slots for lanes
 // 0 and 1).
-template <int S>
-__device__ __forceinline__ void x_tile66(const View& in, const View& out, int j, int i0, int i1, double dtd,
+template <int S, bool F>
+__device__ __forceinline__ bool x_tile66_pass(const View& in, const View& out, int j, int i0, int i1, double dtd,
                                        double half, const Bc& bc, int ghosts, XLds66& X, int lane) {
     int it = 0;
     const int niter = (i1 - i0 + 4 + 63) / 64;
+    unsigned key = 0;   // running range key of the states read (fast pass)
     Cons Un = ld(in, min(i0 - 2 + lane, in.nx + 1), j);
     for (int k = 0; k < niter; ++k) {
-        const int p = i0 - 2 + 64 * k + lane;   // cell loaded by this lane
+        const int p = i0 - 2 + 64 * k + lane;
         const Cons U0 = Un;
         Un = ld(in, min(p + 64, in.nx + 1), j);
-        const Prim W0 = c2p(U0);
+        const Prim W0 = c2p<F>(U0);
+        if constexpr (F) key = max(key, range_key(U0.r, U0.e, W0.p));
         X.W[0][2 + lane] = W0.r; X.W[1][2 + lane] = W0.u; X.W[2][2 + lane] = W0.v; X.W[3][2 + lane] = W0.p;
         X.U[0][2 + lane] = U0.r; X.U[1][2 + lane] = U0.mu; X.U[2][2 + lane] = U0.mv; X.U[3][2 + lane] = U0.e;
         wave_sync();
@@ -196,7 +247,7 @@ __device__ __forceinline__ void x_tile66(const View& in, const View& out, int j,
             X.W[0][lane - 62] = W0.r; X.W[1][lane - 62] = W0.u; X.W[2][lane - 62] = W0.v; X.W[3][lane - 62] = W0.p;
         }
         Edge L, R;   // reconstruct q = p-1
-        reconstruct<kX>(W2, W1, W0, half, L, R);
+        reconstruct<kX, F>(W2, W1, W0, half, L, R);
         X.R[0][2 + lane] = R.U.r; X.R[1][2 + lane] = R.U.mu; X.R[2][2 + lane] = R.U.mv; X.R[3][2 + lane] = R.U.e;
         X.R[4][2 + lane] = R.V.r; X.R[5][2 + lane] = R.V.u; X.R[6][2 + lane] = R.V.v; X.R[7][2 + lane] = R.V.p;
         wave_sync();
@@ -207,18 +258,18 @@ __device__ __forceinline__ void x_tile66(const View& in, const View& out, int j,
             X.R[0][1] = R.U.r; X.R[1][1] = R.U.mu; X.R[2][1] = R.U.mv; X.R[3][1] = R.U.e;
             X.R[4][1] = R.V.r; X.R[5][1] = R.V.u; X.R[6][1] = R.V.v; X.R[7][1] = R.V.p;
         }
-        const Cons F = riemann<S, kX>(RU, RV, L.U, L.V, it);   // face q-1/2 = r+1/2, r = p-2
-        X.F[0][2 + lane] = F.r; X.F[1][2 + lane] = F.mu; X.F[2][2 + lane] = F.mv; X.F[3][2 + lane] = F.e;
+        const Cons Fc = riemann<S, kX, F>(RU, RV, L.U, L.V, it);   // face q-1/2 = r+1/2, r = p-2
+        X.F[0][2 + lane] = Fc.r; X.F[1][2 + lane] = Fc.mu; X.F[2][2 + lane] = Fc.mv; X.F[3][2 + lane] = Fc.e;
         wave_sync();
         const Cons Fm = {X.F[0][1 + lane], X.F[1][1 + lane], X.F[2][1 + lane], X.F[3][1 + lane]};
         const Cons U2 = {X.U[0][lane], X.U[1][lane], X.U[2][lane], X.U[3][lane]};
         wave_sync();
-        if (lane == 63) { X.F[0][1] = F.r; X.F[1][1] = F.mu; X.F[2][1] = F.mv; X.F[3][1] = F.e; }
+        if (lane == 63) { X.F[0][1] = Fc.r; X.F[1][1] = Fc.mu; X.F[2][1] = Fc.mv; X.F[3][1] = Fc.e; }
         if (lane >= 62)
             for (int c = 0; c < 4; ++c) X.U[c][lane - 62] = X.U[c][lane + 2];
         const int r = p - 2;
         if (r >= i0 && r < i1) {
-            const Cons Unew = update(U2, dtd, Fm, F);
+            const Cons Unew = update(U2, dtd, Fm, Fc);
             st(out, r, j, Unew);
             if (ghosts) {
                 if (j == 0) {
@@ -234,21 +285,25 @@ __device__ __forceinline__ void x_tile66(const View& in, const View& out, int j,
             }
         }
     }
+    if constexpr (F) return !__any(key >= kRangeKeyEnd);
+    return true;
 }
 
 // The same x tile over the compact slice.
-template <int S>
-__device__ __forceinline__ void x_tilec(const View& in, const View& out, int j, int i0, int i1, double dtd,
+template <int S, bool F>
+__device__ __forceinline__ bool x_tilec_pass(const View& in, const View& out, int j, int i0, int i1, double dtd,
                                         double half, const Bc& bc, int ghosts, XLdsC& X, int lane) {
     int it = 0;
     const int niter = (i1 - i0 + 4 + 63) / 64;
     const bool l1 = lane >= 1, l2 = lane >= 2;
+    unsigned key = 0;   // running range key of the states read (fast pass)
     Cons Un = ld(in, min(i0 - 2 + lane, in.nx + 1), j);
     for (int k = 0; k < niter; ++k) {
         const int p = i0 - 2 + 64 * k + lane;
         const Cons U0 = Un;
         Un = ld(in, min(p + 64, in.nx + 1), j);
-        const Prim W0 = c2p(U0);
+        const Prim W0 = c2p<F>(U0);
+        if constexpr (F) key = max(key, range_key(U0.r, U0.e, W0.p));
         X.M[0][lane] = W0.r; X.M[1][lane] = W0.u; X.M[2][lane] = W0.v; X.M[3][lane] = W0.p;
         X.U[0][lane] = U0.r; X.U[1][lane] = U0.mu; X.U[2][lane] = U0.mv; X.U[3][lane] = U0.e;
         wave_sync();
@@ -265,7 +320,7 @@ __device__ __forceinline__ void x_tilec(const View& in, const View& out, int j, 
         const Prim W1 = {q1[0], q1[1], q1[2], q1[3]};
         const Prim W2 = {q2[0], q2[1], q2[2], q2[3]};
         Edge L, R;
-        reconstruct<kX>(W2, W1, W0, half, L, R);
+        reconstruct<kX, F>(W2, W1, W0, half, L, R);
         X.M[0][lane] = R.U.r; X.M[1][lane] = R.U.mu; X.M[2][lane] = R.U.mv; X.M[3][lane] = R.U.e;
         X.M[4][lane] = R.V.r; X.M[5][lane] = R.V.u; X.M[6][lane] = R.V.v; X.M[7][lane] = R.V.p;
         wave_sync();
@@ -278,8 +333,8 @@ __device__ __forceinline__ void x_tilec(const View& in, const View& out, int j, 
         }
         const Cons RU = {r1[0], r1[1], r1[2], r1[3]};
         const Prim RV = {r1[4], r1[5], r1[6], r1[7]};
-        const Cons F = riemann<S, kX>(RU, RV, L.U, L.V, it);
-        X.M[0][lane] = F.r; X.M[1][lane] = F.mu; X.M[2][lane] = F.mv; X.M[3][lane] = F.e;
+        const Cons Fc = riemann<S, kX, F>(RU, RV, L.U, L.V, it);
+        X.M[0][lane] = Fc.r; X.M[1][lane] = Fc.mu; X.M[2][lane] = Fc.mv; X.M[3][lane] = Fc.e;
         wave_sync();
         double f1[4], u2[4];
         for (int c = 0; c < 4; ++c) {
@@ -287,14 +342,14 @@ __device__ __forceinline__ void x_tilec(const View& in, const View& out, int j, 
             u2[c] = l2 ? X.U[c][lane - 2] : X.cU[c][lane];
         }
         wave_sync();
-        if (lane == 63) { X.cF[0] = F.r; X.cF[1] = F.mu; X.cF[2] = F.mv; X.cF[3] = F.e; }
+        if (lane == 63) { X.cF[0] = Fc.r; X.cF[1] = Fc.mu; X.cF[2] = Fc.mv; X.cF[3] = Fc.e; }
         if (lane >= 62)
             for (int c = 0; c < 4; ++c) X.cU[c][lane - 62] = X.U[c][lane];
         const Cons Fm = {f1[0], f1[1], f1[2], f1[3]};
         const Cons U2 = {u2[0], u2[1], u2[2], u2[3]};
         const int r = p - 2;
         if (r >= i0 && r < i1) {
-            const Cons Unew = update(U2, dtd, Fm, F);
+            const Cons Unew = update(U2, dtd, Fm, Fc);
             st(out, r, j, Unew);
             if (ghosts) {
                 if (j == 0) {
@@ -310,6 +365,8 @@ __device__ __forceinline__ void x_tilec(const View& in, const View& out, int j, 
             }
         }
     }
+    if constexpr (F) return !__any(key >= kRangeKeyEnd);
+    return true;
 }
 
 // Layout selector: L = 0 compact, 1 = 66-slot.
@@ -320,8 +377,15 @@ template <int S, int L>
 __device__ __forceinline__ void x_tile(const View& in, const View& out, int j, int i0, int i1, double dtd,
                                        double half, const Bc& bc, int ghosts, typename XSlice<L>::type& X,
                                        int lane) {
-    if constexpr (L == 1) x_tile66<S>(in, out, j, i0, i1, dtd, half, bc, ghosts, X, lane);
-    else x_tilec<S>(in, out, j, i0, i1, dtd, half, bc, ghosts, X, lane);
+    // fast pass first (see y_tile_pass); the IEEE pass redoes the segment if
+    // the fast one gives up
+    if constexpr (L == 1) {
+        if (dt_in_fast_range(dtd) && x_tile66_pass<S, true>(in, out, j, i0, i1, dtd, half, bc, ghosts, X, lane)) return;
+        x_tile66_pass<S, false>(in, out, j, i0, i1, dtd, half, bc, ghosts, X, lane);
+    } else {
+        if (dt_in_fast_range(dtd) && x_tilec_pass<S, true>(in, out, j, i0, i1, dtd, half, bc, ghosts, X, lane)) return;
+        x_tilec_pass<S, false>(in, out, j, i0, i1, dtd, half, bc, ghosts, X, lane);
+    }
 }
 
 // Host helpers defined in solver.hip.

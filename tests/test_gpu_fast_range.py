@@ -1,0 +1,175 @@
+"""The sweeps' two arithmetic paths give the same bits.
+
+A tile whose states all lie in the range where the unscaled division and
+square root sequences are exact (rho and p in [2^-20, 2^20], |rho u|,
+|rho v|, E <= 2^20, dt/d in [2^-20, 2^20]) keeps the result of its fast
+pass; any other tile is redone with the IEEE sequences.  These runs put
+states on both sides of that range in one grid (density and pressure scaled
+smoothly by up to 10^7 and down to 10^-7, velocities unchanged), use a dt
+outside it, and hold every result to the oracle bit for bit (HLL, HLLC,
+FORCE) -- in single sweeps, in the device loop whose dt comes from the
+fused signal-speed maximum, and in the batched ensemble.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from conftest import interior
+
+pytestmark = pytest.mark.gpu
+
+BITWISE_SOLVERS = [0, 1, 3]
+
+
+def scaled_field(oracle, nx, ny, seed, decades=7.0):
+    """A smooth field with two blobs: conserved state x 10^decades around one
+    centre, x 10^-decades around another (so rho and p leave the fast range
+    there; sound speeds do not change)."""
+    rng = np.random.default_rng(seed)
+    y, x = np.mgrid[0:ny + 4, 0:nx + 4]
+    x, y = (x - 1.5) / nx, (y - 1.5) / ny
+    rho = 1.0 + 0.3 * np.sin(7 * x) * np.cos(5 * y)
+    u = 0.4 * np.cos(3 * y)
+    v = 0.3 * np.sin(4 * x)
+    p = 1.0 + 0.1 * rng.random((ny + 4, nx + 4))
+    U = oracle.prim_to_cons(np.stack([rho, u, v, p], axis=-1).reshape(-1, 4)).reshape(ny + 4, nx + 4, 4)
+    r_hi = (x - 0.3) ** 2 + (y - 0.3) ** 2
+    r_lo = (x - 0.75) ** 2 + (y - 0.7) ** 2
+    s = 10.0 ** (decades * (np.exp(-r_hi / 0.03) - np.exp(-r_lo / 0.03)))
+    return U * s[..., None]
+
+
+def in_fast_range_fraction(U):
+    lo, hi = 2.0 ** -20, 2.0 ** 20
+    rho, mu, mv, e = (U[..., c] for c in range(4))
+    p = 0.4 * (e - 0.5 * (mu * mu + mv * mv) / rho)
+    ok = (rho >= lo) & (rho <= hi) & (p >= lo) & (p <= hi) & (np.abs(mu) <= hi) & (np.abs(mv) <= hi) & (e <= hi)
+    return ok.mean()
+
+
+def test_scaled_field_spans_both_paths(oracle):
+    for nx, ny in ((200, 150), (67, 300), (300, 260)):
+        assert 0.85 < in_fast_range_fraction(scaled_field(oracle, nx, ny, 0)) < 0.98
+    assert in_fast_range_fraction(scaled_field(oracle, 130, 97, 0, decades=0.0)) == 1.0
+
+
+@pytest.mark.parametrize("solver", BITWISE_SOLVERS)
+@pytest.mark.parametrize("direction", [0, 1])
+@pytest.mark.parametrize("shape", [(200, 150), (67, 300)])
+def test_sweep_mixed_range_bitwise(hip, oracle, solver, direction, shape):
+    from large_sys_eq_opt_amd import Grid2DGPU, sweep_gpu
+
+    nx, ny = shape
+    U = scaled_field(oracle, nx, ny, nx + ny)
+    dx, dy = 1.0 / nx, 1.0 / ny
+    dt = 0.2 * min(dx, dy)
+    ref = oracle.sweep(U, dx, dy, dt, direction, solver)
+    a = Grid2DGPU(nx, ny, 2, 0.0, 1.0, 0.0, 1.0)
+    b = Grid2DGPU(nx, ny, 2, 0.0, 1.0, 0.0, 1.0)
+    a.upload_from_aos(U)
+    b.upload_from_aos(U)
+    sweep_gpu(a, b, dt, direction, solver)
+    assert np.array_equal(interior(b.download_to_aos()), interior(ref))
+
+
+@pytest.mark.parametrize("solver", BITWISE_SOLVERS)
+@pytest.mark.parametrize("direction", [0, 1])
+@pytest.mark.parametrize("dt", [1e-9, 2e4])
+def test_sweep_dt_outside_fast_range(hip, oracle, solver, direction, dt):
+    """dt/d = 1.3e-7 and 2.6e6: the IEEE pass only (the large step gives
+    non-physical values, which must still match, NaN for NaN)."""
+    from large_sys_eq_opt_amd import Grid2DGPU, sweep_gpu
+
+    nx, ny = 130, 97
+    U = scaled_field(oracle, nx, ny, 5, decades=0.0)   # states in range: only dt is outside
+    dx, dy = 1.0 / nx, 1.0 / ny
+    ref = oracle.sweep(U, dx, dy, dt, direction, solver)
+    a = Grid2DGPU(nx, ny, 2, 0.0, 1.0, 0.0, 1.0)
+    b = Grid2DGPU(nx, ny, 2, 0.0, 1.0, 0.0, 1.0)
+    a.upload_from_aos(U)
+    b.upload_from_aos(U)
+    sweep_gpu(a, b, dt, direction, solver)
+    assert np.array_equal(interior(b.download_to_aos()), interior(ref), equal_nan=True)
+
+
+SHAPES = [
+    dict(seg_x=1020, seg_y=128, y_waves=1, x_layout=0, x_wpb=1, y_wpb=4, min_tiles=0),
+    dict(seg_x=60, seg_y=16, y_waves=4, x_layout=1, x_wpb=4, y_wpb=1, min_tiles=0),
+    dict(seg_x=1020, seg_y=128, y_waves=1, x_layout=0, x_wpb=1, y_wpb=4, min_tiles=2048),
+]
+
+
+@pytest.mark.parametrize("solver", BITWISE_SOLVERS)
+@pytest.mark.parametrize("k", range(len(SHAPES)))
+def test_device_loop_mixed_range_bitwise(hip, oracle, solver, k):
+    """Ten device-loop steps on a ragged 300x260 grid: the dt of each step
+    comes from the signal-speed maximum fused into the y sweep, which the
+    redone tiles feed too, so the dt sequence checks that path."""
+    from large_sys_eq_opt_amd import Grid2DGPU, Stepper, cases
+    from large_sys_eq_opt_amd.solver import get_launch_shape, set_launch_shape
+
+    nx, ny, steps = 300, 260, 10
+    cfg = cases.CaseConfig(nx, ny, 2, 0.0, 1.0, 0.0, 1.0, 0.8, 0.0)
+    U0 = scaled_field(oracle, nx, ny, 11)
+    o = oracle.run(U0, cfg.dx, cfg.dy, solver, cfg.cfl, max_steps=steps)
+    saved = get_launch_shape()
+    try:
+        set_launch_shape(**SHAPES[k])
+        g = Grid2DGPU.from_config(cfg)
+        g.upload_from_aos(U0)
+        st = Stepper(g, solver, cfg.cfl, 0.0, None, log_capacity=steps)
+        st.advance(steps)
+        st.finish()
+        _, n, dts = st.clock()
+        out = g.download_to_aos()
+    finally:
+        set_launch_shape(**saved)
+    assert n == steps
+    assert np.array_equal(dts, o["dt"])
+    assert np.array_equal(interior(out), interior(o["U"]))
+
+
+def test_host_loop_mixed_range_bitwise(hip, oracle):
+    """The host-synchronous step (dt from mh_compute_dt) on the same field."""
+    from large_sys_eq_opt_amd import Grid2DGPU, advance_second_order_gpu, compute_dt_gpu
+    from large_sys_eq_opt_amd.solver import apply_transmissive_boundary_gpu
+
+    nx, ny, steps = 200, 150, 6
+    U0 = scaled_field(oracle, nx, ny, 3)
+    o = oracle.run(U0, 1.0 / nx, 1.0 / ny, 1, 0.8, max_steps=steps)
+    U, T, N = (Grid2DGPU(nx, ny, 2, 0.0, 1.0, 0.0, 1.0) for _ in range(3))
+    U.upload_from_aos(U0)
+    apply_transmissive_boundary_gpu(U)   # the oracle's loop fills the ghosts first
+    dts = []
+    for _ in range(steps):
+        dt = compute_dt_gpu(U, None, 0.8)
+        dts.append(dt)
+        advance_second_order_gpu(U, T, N, None, dt, 1)
+        U.swap(N)
+    assert np.array_equal(np.array(dts), o["dt"])
+    assert np.array_equal(interior(U.download_to_aos()), interior(o["U"]))
+
+
+@pytest.mark.parametrize("schedule", [(0, 256, 0, 0, 1), (0, 64, 2, 0, 1), (60, 16, 2, 0, 3)])
+def test_batch_out_of_range_instance(hip, oracle, schedule):
+    """An ensemble where one blast's inner pressure (5e6) puts its energy
+    above 2^20 while the others stay in range; every instance still equals
+    the oracle."""
+    from large_sys_eq_opt_amd.ensemble import Batch, blast_params
+
+    spec = [{"side": 96, "r0": 0.3, "p_in": 5e6}, {"side": 64, "r0": 0.4, "p_in": 10.0},
+            {"side": 130, "r0": 0.2, "p_in": 2e6}]
+    steps = 8
+    b = Batch(spec, solver=1)
+    b.set_schedule(*schedule)
+    b.initialise()
+    b.step(steps)
+    _, _, n, status = b.clock()
+    assert (n == steps).all() and (status == 0).all()
+    for k, s in enumerate(spec):
+        side = s["side"]
+        U0 = oracle.init(2, blast_params(s), side, side, (0.0, 2.0, 0.0, 2.0))
+        o = oracle.run(U0, 2.0 / side, 2.0 / side, 1, 0.8, max_steps=steps)
+        assert np.array_equal(interior(b.download(k)), interior(o["U"])), (k, s)
+    b.close()
