@@ -24,8 +24,8 @@
 // Step parity, counters and clocks live in device memory, so the launch is
 // identical every step and a captured graph of k launches replays k steps.
 // The schedule can instead use two launches per step (x, then y), either of
-// the same persistent kernel or of two dispatched kernels (k_batch_x,
-// k_batch_y) with one tile per wavefront: there the hardware dispatcher
+// the same persistent kernel or of dispatched kernels (k_batch_x, k_batch_y,
+// k_batch_y_redo) with one tile per wavefront: there the hardware dispatcher
 // balances the load as tiles finish, tiles are ordered largest instance
 // first, and each phase gets its own register budget.  The autotuner times
 // the modes.
@@ -75,6 +75,9 @@ struct BatchArgs {
     const int* ystart;   // nwaves + 1
     Ctrl* ctrl;
     int phases;          // 1 = x only, 2 = y only, 3 = x, barrier, y
+    // dispatched y phase: y tiles whose fast pass gave up, left to
+    // k_batch_y_redo ([0] count, [1] finished workgroups, [2..] tile indices)
+    int* redo;
 };
 
 __device__ __forceinline__ bool inst_dt(const BatchInst& I, const BatchClock& c, int par, double& dt) {
@@ -190,32 +193,63 @@ __global__ __launch_bounds__(64) void k_batch_x(BatchArgs a, int ntiles) {
     for (int j = t.j0; j < t.j1; ++j) x_tile<S, 0>(I.U, I.T, j, t.i0, t.i1, dtd, 0.5 * dtd, I.bc, 1, X[w], lane);
 }
 
-// Dispatched y phase: wavefront q runs y tile q; the last workgroup to
-// finish advances the step counter.
+// Dispatched y phase: wavefront q runs y tile q with the fast division
+// (y_tile_pass<S, true>); a tile whose states leave the fast range goes on
+// the redo list for k_batch_y_redo, launched right after, which also
+// advances the step counter.  Keeping the IEEE pass out of this kernel keeps
+// it within the register budget of four wavefronts per SIMD.
 template <int S>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 8))) void k_batch_y(BatchArgs a, int ntiles) {
     __shared__ double ring[4][kRing];
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const int par = __hip_atomic_load(&a.ctrl->step, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & 1;
+    const int q = blockIdx.x * 4 + w;
+    if (q >= ntiles) return;
+    const YTile t = a.yt[q];
+    // the instance's fields are read once here, before any store, so they
+    // are scalar loads kept in scalar registers
+    const BatchInst I = a.inst[t.inst];
+    double dt;
+    if (!inst_dt(I, a.clk[t.inst], par, dt)) return;
+    const double dtd = dt / I.dy;
+    double m = 0.0;
+    if (dt_in_fast_range(dtd) &&
+        y_tile_pass<S, true>(I.T, I.U, t.cg, t.j0, t.j1, dtd, 0.5 * dtd, I.bc, 1, 1, ring[w], lane, m)) {
+        m = wave_max(m);
+        if (lane == 0 && m > 0.0) atomicMax(&a.clk[t.inst].smax[par ^ 1], (unsigned long long)__double_as_longlong(m));
+    } else if (lane == 0) {
+        a.redo[2 + atomicAdd(a.redo, 1)] = q;
+    }
+}
+
+// IEEE pass over the y tiles k_batch_y listed; the last workgroup to finish
+// empties the list and advances the step counter.
+template <int S>
+__global__ __launch_bounds__(256) void k_batch_y_redo(BatchArgs a) {
+    __shared__ double ring[4][kRing];
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     const int step = __hip_atomic_load(&a.ctrl->step, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     const int par = step & 1;
-    const int q = blockIdx.x * 4 + w;
-    if (q < ntiles) {
-        const YTile t = a.yt[q];
+    const int n = __hip_atomic_load(a.redo, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    for (int r = blockIdx.x * 4 + w; r < n; r += gridDim.x * 4) {
+        const YTile t = a.yt[a.redo[2 + r]];
         const BatchInst& I = a.inst[t.inst];
         double dt;
-        if (inst_dt(I, a.clk[t.inst], par, dt)) {
-            const double dtd = dt / I.dy;
-            double m = y_tile<S>(I.T, I.U, t.cg, t.j0, t.j1, dtd, 0.5 * dtd, I.bc, 1, 1, ring[w], lane);
-            m = wave_max(m);
-            if (lane == 0 && m > 0.0)
-                atomicMax(&a.clk[t.inst].smax[par ^ 1], (unsigned long long)__double_as_longlong(m));
-        }
+        if (!inst_dt(I, a.clk[t.inst], par, dt)) continue;
+        const double dtd = dt / I.dy;
+        double m = 0.0;
+        y_tile_pass<S, false>(I.T, I.U, t.cg, t.j0, t.j1, dtd, 0.5 * dtd, I.bc, 1, 1, ring[w], lane, m);
+        m = wave_max(m);
+        if (lane == 0 && m > 0.0) atomicMax(&a.clk[t.inst].smax[par ^ 1], (unsigned long long)__double_as_longlong(m));
     }
     __syncthreads();
     if (threadIdx.x == 0) {
         __threadfence();
-        if (atomicAdd(&a.ctrl->done[par], 1u) == gridDim.x - 1)
+        if (atomicAdd(a.redo + 1, 1) == (int)gridDim.x - 1) {
+            a.redo[0] = 0;
+            a.redo[1] = 0;
             __hip_atomic_store(&a.ctrl->step, step + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
     }
 }
 
@@ -254,6 +288,7 @@ struct mh_batch_s {
     YTile* dyt = nullptr;
     int* dxs = nullptr;
     int* dys = nullptr;
+    int* dredo = nullptr;   // dispatched y phase's redo list (2 + nyt ints)
     int nblocks = 0;
     int nxt = 0, nyt = 0;
     // schedule; mode 0: one persistent launch per step, 1: two persistent
@@ -367,12 +402,14 @@ int build_schedule(mh_batch b) {
         lpt(yt, yc);
     }
     const std::vector<int> xs = partition(xc, nw), ys = partition(yc, nw);
-    free_dev(b->dxt); free_dev(b->dyt); free_dev(b->dxs); free_dev(b->dys);
-    b->dxt = nullptr; b->dyt = nullptr; b->dxs = nullptr; b->dys = nullptr;
+    free_dev(b->dxt); free_dev(b->dyt); free_dev(b->dxs); free_dev(b->dys); free_dev(b->dredo);
+    b->dxt = nullptr; b->dyt = nullptr; b->dxs = nullptr; b->dys = nullptr; b->dredo = nullptr;
     rc = check_hip(hipMalloc((void**)&b->dxt, sizeof(XTile) * xt.size()), "hipMalloc(tiles)");
     if (!rc) rc = check_hip(hipMalloc((void**)&b->dyt, sizeof(YTile) * yt.size()), "hipMalloc(tiles)");
     if (!rc) rc = check_hip(hipMalloc((void**)&b->dxs, sizeof(int) * xs.size()), "hipMalloc(tiles)");
     if (!rc) rc = check_hip(hipMalloc((void**)&b->dys, sizeof(int) * ys.size()), "hipMalloc(tiles)");
+    if (!rc) rc = check_hip(hipMalloc((void**)&b->dredo, sizeof(int) * (2 + yt.size())), "hipMalloc(tiles)");
+    if (!rc) rc = check_hip(hipMemset(b->dredo, 0, 2 * sizeof(int)), "hipMemset");
     if (!rc) rc = check_hip(hipMemcpy(b->dxt, xt.data(), sizeof(XTile) * xt.size(), hipMemcpyHostToDevice), "hipMemcpy");
     if (!rc) rc = check_hip(hipMemcpy(b->dyt, yt.data(), sizeof(YTile) * yt.size(), hipMemcpyHostToDevice), "hipMemcpy");
     if (!rc) rc = check_hip(hipMemcpy(b->dxs, xs.data(), sizeof(int) * xs.size(), hipMemcpyHostToDevice), "hipMemcpy");
@@ -392,18 +429,24 @@ int launch_step(mh_batch b, hipStream_t st) {
     a.yt = b->dyt;
     a.ystart = b->dys;
     a.ctrl = b->dctrl;
+    a.redo = b->dredo;
     const dim3 g(b->nblocks), blk(256);
     if (b->mode == 2) {
         const dim3 gx((unsigned)std::max(1, b->nxt)), gy((unsigned)std::max(1, (b->nyt + 3) / 4)), bx(64);
+        const dim3 gr((unsigned)std::max(1, std::min(64, (b->nyt + 3) / 4)));
         switch (b->solver) {
             case kHLL: hipLaunchKernelGGL(k_batch_x<kHLL>, gx, bx, 0, st, a, b->nxt);
-                       hipLaunchKernelGGL(k_batch_y<kHLL>, gy, blk, 0, st, a, b->nyt); break;
+                       hipLaunchKernelGGL(k_batch_y<kHLL>, gy, blk, 0, st, a, b->nyt);
+                       hipLaunchKernelGGL(k_batch_y_redo<kHLL>, gr, blk, 0, st, a); break;
             case kHLLC: hipLaunchKernelGGL(k_batch_x<kHLLC>, gx, bx, 0, st, a, b->nxt);
-                        hipLaunchKernelGGL(k_batch_y<kHLLC>, gy, blk, 0, st, a, b->nyt); break;
+                        hipLaunchKernelGGL(k_batch_y<kHLLC>, gy, blk, 0, st, a, b->nyt);
+                        hipLaunchKernelGGL(k_batch_y_redo<kHLLC>, gr, blk, 0, st, a); break;
             case kExact: hipLaunchKernelGGL(k_batch_x<kExact>, gx, bx, 0, st, a, b->nxt);
-                         hipLaunchKernelGGL(k_batch_y<kExact>, gy, blk, 0, st, a, b->nyt); break;
+                         hipLaunchKernelGGL(k_batch_y<kExact>, gy, blk, 0, st, a, b->nyt);
+                         hipLaunchKernelGGL(k_batch_y_redo<kExact>, gr, blk, 0, st, a); break;
             default: hipLaunchKernelGGL(k_batch_x<kForce>, gx, bx, 0, st, a, b->nxt);
-                     hipLaunchKernelGGL(k_batch_y<kForce>, gy, blk, 0, st, a, b->nyt); break;
+                     hipLaunchKernelGGL(k_batch_y<kForce>, gy, blk, 0, st, a, b->nyt);
+                     hipLaunchKernelGGL(k_batch_y_redo<kForce>, gr, blk, 0, st, a); break;
         }
         return check_hip(hipGetLastError(), "batch step launch");
     }
@@ -493,7 +536,7 @@ int mh_batch_destroy(mh_batch b) {
     if (!b) return MH_OK;
     drop_graph(b);
     free_dev(b->arena); free_dev(b->dinst); free_dev(b->dclk); free_dev(b->dctrl);
-    free_dev(b->dxt); free_dev(b->dyt); free_dev(b->dxs); free_dev(b->dys);
+    free_dev(b->dxt); free_dev(b->dyt); free_dev(b->dxs); free_dev(b->dys); free_dev(b->dredo);
     delete b;
     return MH_OK;
 }
@@ -514,6 +557,7 @@ int mh_batch_prepare(mh_batch b, mh_stream stream) {
     Ctrl c0{};
     if (!rc) rc = check_hip(hipMemcpyAsync(b->dclk, clk.data(), sizeof(BatchClock) * b->n, hipMemcpyHostToDevice, st), "hipMemcpyAsync");
     if (!rc) rc = check_hip(hipMemcpyAsync(b->dctrl, &c0, sizeof(Ctrl), hipMemcpyHostToDevice, st), "hipMemcpyAsync");
+    if (!rc) rc = check_hip(hipMemsetAsync(b->dredo, 0, 2 * sizeof(int), st), "hipMemsetAsync");
     int maxcells = 0;
     for (auto& I : b->hinst) maxcells = std::max(maxcells, I.U.nx * I.U.ny);
     if (!rc) {
