@@ -33,6 +33,16 @@ __device__ __forceinline__ void st(const View& v, int i, int j, const Cons& U) {
     q[3 * v.ps] = U.e;
 }
 
+// Every lane of a tile issues its stores, also a lane with no cell to write:
+// that lane writes into column kSinkCol of the row, which lies in the row's
+// lead padding (left of the two ghost columns) and is never read.  With no
+// branch around the stores, the loads prefetched before them can be waited
+// for without waiting for the stores (the vector memory counter counts both
+// and completes in order: a skipped store block would leave the load the
+// newest event and force a full wait every row).
+constexpr int kSinkCol = -kNg - 1;
+static_assert(kLead >= kNg + 1, "the sink column must lie in the lead padding");
+
 // Boundary spec in kernel-argument form.
 struct Bc {
     int kind[4];
@@ -169,8 +179,8 @@ __device__ __forceinline__ bool y_tile_pass(const View& in, const View& out, int
         const int r = p - 2;
         const Cons Unew = update(get(slot_old), dtd, Fp, Fc);
         put(slot_new, U0);
+        st(out, active ? i : kSinkCol, r, Unew);
         if (active) {
-            st(out, i, r, Unew);
             // From rows in range, the new state's rho and p are zero or within
             // about 2^-150..2^400 (differences of in-range values cannot fall
             // below their unit in the last place), so its signal speed's
@@ -268,9 +278,10 @@ __device__ __forceinline__ bool x_tile66_pass(const View& in, const View& out, i
         if (lane >= 62)
             for (int c = 0; c < 4; ++c) X.U[c][lane - 62] = X.U[c][lane + 2];
         const int r = p - 2;
-        if (r >= i0 && r < i1) {
-            const Cons Unew = update(U2, dtd, Fm, Fc);
-            st(out, r, j, Unew);
+        const bool mine = r >= i0 && r < i1;
+        const Cons Unew = update(U2, dtd, Fm, Fc);
+        st(out, mine ? r : kSinkCol, j, Unew);
+        if (mine) {
             if (ghosts) {
                 if (j == 0) {
                     const Cons G = bc.kind[MH_EDGE_BOTTOM] == MH_BC_INFLOW ? bc.s[MH_EDGE_BOTTOM] : Unew;
@@ -348,9 +359,10 @@ __device__ __forceinline__ bool x_tilec_pass(const View& in, const View& out, in
         const Cons Fm = {f1[0], f1[1], f1[2], f1[3]};
         const Cons U2 = {u2[0], u2[1], u2[2], u2[3]};
         const int r = p - 2;
-        if (r >= i0 && r < i1) {
-            const Cons Unew = update(U2, dtd, Fm, Fc);
-            st(out, r, j, Unew);
+        const bool mine = r >= i0 && r < i1;
+        const Cons Unew = update(U2, dtd, Fm, Fc);
+        st(out, mine ? r : kSinkCol, j, Unew);
+        if (mine) {
             if (ghosts) {
                 if (j == 0) {
                     const Cons G = bc.kind[MH_EDGE_BOTTOM] == MH_BC_INFLOW ? bc.s[MH_EDGE_BOTTOM] : Unew;
