@@ -175,7 +175,7 @@ def main():
     ap.add_argument("--no-extras", action="store_true", help="skip the batched and adjoint legs")
     ap.add_argument("--leg", choices=["batched", "adjoint"], default=None,
                     help="run only this extra leg (batched: against --plain-rate Gcell/s)")
-    ap.add_argument("--plain-rate", type=float, default=28.0)
+    ap.add_argument("--plain-rate", type=float, default=32.0)
     ap.add_argument("--copy-only", type=int, default=0,
                     help="only run the copy kernel on buffers of this many MiB (counter calibration)")
     args = ap.parse_args()

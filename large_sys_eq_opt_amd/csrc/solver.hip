@@ -113,7 +113,9 @@ struct SweepArgs {
 // output grid's redo list and left to k_sweep_y_redo, which the host
 // launches right after on the same stream (it adds the tile's speed
 // maximum); with RL = 0 the IEEE pass runs in place.  Keeping the IEEE pass
-// out of this kernel keeps its hot loop lean.
+// out of the kernel keeps it within 128 VGPRs (four wavefronts per SIMD);
+// small grids, where the second launch would cost more than occupancy
+// gains, use RL = 0.
 __device__ __forceinline__ unsigned long long* speed_slot(const SweepArgs& a) {
     return a.clock_speed ? &a.dts.clock->smax[a.dts.clock->slot ^ 1] : a.speed;
 }
@@ -490,7 +492,10 @@ int launch_sweep(const mh_grid_s* in, const mh_grid_s* out, int dir, int solver,
         a.seg = seg_y_for(in);
         a.ngroups = (in->d.nx + 63) / 64;
         a.ntiles = a.ngroups * ((in->d.ny + a.seg - 1) / a.seg);
-        if (out->redo && a.ntiles <= out->redo_cap) a.redo = out->redo;
+        // The redo list costs a second launch: worth it only where the sweep
+        // is throughput-bound (at least min_tiles tiles); small, latency-bound
+        // grids take the IEEE pass in place.
+        if (out->redo && a.ntiles >= g_shape.min_tiles && a.ntiles <= out->redo_cap) a.redo = out->redo;
         const int wpb = a.ntiles < g_shape.min_tiles ? 1 : g_shape.y_wpb;
         const dim3 g((unsigned)((a.ntiles + wpb - 1) / wpb));
         const int W = g_shape.y_waves;
