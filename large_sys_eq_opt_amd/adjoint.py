@@ -1,5 +1,6 @@
-"""Discrete adjoint of the split MUSCL-Hancock step (HLL / HLLC) and its
-PyTorch binding.
+"""Discrete adjoint of the split MUSCL-Hancock step (HLL, HLLC and the exact
+solver; for the exact solver, the implicit derivative of p* at the converged
+Newton root) and its PyTorch binding.
 
 ``Adjoint`` wraps the C ABI workspace (csrc/adjoint.hip): the vector-Jacobian
 product of one sweep or one split step at a given dt, and N-step

@@ -1,6 +1,7 @@
 // adjoint_device.h -- hand-written reverse mode (vector-Jacobian products)
 // of the device physics in euler_device.h: conversions, physical flux, sound
-// speed, HLL and HLLC with every branch, and the MUSCL-Hancock
+// speed, HLL and HLLC with every branch, the exact solver (implicit
+// derivative of p* and every sampling branch), and the MUSCL-Hancock
 // reconstruction with its limiter and fallbacks.
 //
 // Conventions.  `b` prefixes an adjoint; every *_vjp accumulates (+=) into
@@ -249,10 +250,170 @@ __device__ void hllc_vjp(const Cons& UL, const Prim& VL, const Cons& UR, const P
     speeds_vjp<D>(VL, VR, aL, aR, bSL, bSR, bVL, bVR);
 }
 
+// ---------------------------------------------------------------------------
+// Exact solver (head/riemann.hpp:331-644)
+// ---------------------------------------------------------------------------
+// Partial derivatives of the pressure function f_K(p; rho_K, p_K, a_K) of
+// exact_pfun (a_K treated as its own variable; the caller chains it through
+// the sound speed).
+__device__ __forceinline__ void exact_pfun_partials(double p, double rhoK, double pK, double aK, double& fp,
+                                                    double& frho, double& fpK, double& faK) {
+    constexpr double g = kGamma;
+    if (p > pK) {   // shock: f = (p - pK) sqrt(A / (p + B)), A = 2/((g+1) rhoK), B = (g-1)/(g+1) pK
+        const double A = 2.0 / ((g + 1.0) * rhoK);
+        const double B = (g - 1.0) / (g + 1.0) * pK;
+        const double q = sqrt(A / (p + B));
+        const double dq = -0.5 * q / (p + B);   // dq/dp = dq/dB
+        fp = q + (p - pK) * dq;
+        fpK = -q + (p - pK) * dq * ((g - 1.0) / (g + 1.0));
+        frho = (p - pK) * (-0.5 * q / rhoK);
+        faK = 0.0;
+    } else {        // rarefaction: f = 2 aK/(g-1) ((p/pK)^e - 1), e = (g-1)/(2g)
+        constexpr double e = (g - 1.0) / (2.0 * g);
+        const double pr = p / pK;
+        const double dpr = 2.0 * aK / (g - 1.0) * e * pow(pr, e - 1.0);
+        fp = dpr / pK;
+        fpK = -dpr * p / (pK * pK);
+        frho = 0.0;
+        faK = 2.0 / (g - 1.0) * (pow(pr, e) - 1.0);
+    }
+}
+
+// Adjoint of exact_sample_nt: (brho, bun, bp) of the sampled state to the
+// side's primitive state (rho, un, p), its sound speed, p* and u*.  The
+// branch is the one the primal takes (the wave-speed tests decide it).
+template <int D>
+__device__ void exact_sample_vjp(const Prim& VK, double aK, double ps, double us, int side, double brho, double bun,
+                                 double bp, Prim& bVK, double& baK, double& bps, double& bus) {
+    constexpr double g = kGamma;
+    constexpr double g6 = (g - 1.0) / (g + 1.0);
+    const double rhoK = VK.r, pK = VK.p, unK = un_of<D>(VK);
+    double brhoK = 0.0, bunK = 0.0, bpK = 0.0;
+    const double pr = ps / pK;
+    auto ratio_back = [&](double bpr) {   // pr = ps / pK
+        bps += bpr / pK;
+        bpK -= bpr * ps / (pK * pK);
+    };
+    auto shock_star = [&]() {   // rho = rhoK (pr + g6) / (g6 pr + 1), un = u*, p = p*
+        const double N = pr + g6, Dn = g6 * pr + 1.0;
+        brhoK += brho * (N / Dn);
+        ratio_back(brho * rhoK * (1.0 - g6 * g6) / (Dn * Dn));
+        bus += bun;
+        bps += bp;
+    };
+    auto rare_star = [&]() {    // rho = rhoK pr^(1/g), un = u*, p = p*
+        brhoK += brho * pow(pr, 1.0 / g);
+        ratio_back(brho * rhoK * (1.0 / g) * pow(pr, 1.0 / g - 1.0));
+        bus += bun;
+        bps += bp;
+    };
+    auto keep = [&]() { brhoK += brho; bunK += bun; bpK += bp; };
+    // fan at xi = 0: a = 2/(g+1) (aK + s (g-1)/2 unK), un = 2/(g+1) (s aK + (g-1)/2 unK),
+    // s = +1 on the left, -1 on the right; rho = rhoK (a/aK)^(2/(g-1)), p = pK (a/aK)^(2g/(g-1))
+    auto fan = [&](double sgn) {
+        const double a = 2.0 / (g + 1.0) * (aK + sgn * 0.5 * (g - 1.0) * unK);
+        const double ar = a / aK;
+        constexpr double m = 2.0 / (g - 1.0), n = 2.0 * g / (g - 1.0);
+        brhoK += brho * pow(ar, m);
+        bpK += bp * pow(ar, n);
+        const double bar = brho * rhoK * m * pow(ar, m - 1.0) + bp * pK * n * pow(ar, n - 1.0);
+        const double ba = bar / aK;
+        baK += -bar * a / (aK * aK) + ba * (2.0 / (g + 1.0)) + bun * sgn * (2.0 / (g + 1.0));
+        bunK += ba * sgn * ((g - 1.0) / (g + 1.0)) + bun * ((g - 1.0) / (g + 1.0));
+    };
+    if (side == 0) {
+        if (ps > pK) {
+            const double SL = unK - aK * sqrt((g + 1.0) / (2.0 * g) * pr + (g - 1.0) / (2.0 * g));
+            if (!(0.0 <= SL)) shock_star(); else keep();
+        } else {
+            const double a_star = aK * pow(pr, (g - 1.0) / (2.0 * g));
+            if (0.0 <= unK - aK) keep();
+            else if (0.0 >= us - a_star) rare_star();
+            else fan(1.0);
+        }
+    } else {
+        if (ps > pK) {
+            const double SR = unK + aK * sqrt((g + 1.0) / (2.0 * g) * pr + (g - 1.0) / (2.0 * g));
+            if (!(0.0 >= SR)) shock_star(); else keep();
+        } else {
+            const double a_star = aK * pow(pr, (g - 1.0) / (2.0 * g));
+            if (0.0 >= unK + aK) keep();
+            else if (0.0 <= us + a_star) rare_star();
+            else fan(-1.0);
+        }
+    }
+    bVK.r += brhoK;
+    bVK.p += bpK;
+    add_un<D>(bVK, bunK);
+}
+
+// F = pflux(U*, V*) of the state sampled at xi = 0.  p* is the root of
+// f(p) = f_L(p) + f_R(p) + unR - unL: its derivative is the implicit one,
+// dp* = -(df/dq) / (df/dp) at the converged p* (the Newton iterations
+// themselves are not differentiated), and u* = (unL + unR + f_R(p*) -
+// f_L(p*)) / 2 is differentiated through p*.
+template <int D>
+__device__ void exact_vjp(const Cons& UL, const Prim& VL, const Cons& UR, const Prim& VR, const Cons& bF, Cons& bUL,
+                          Prim& bVL, Cons& bUR, Prim& bVR) {
+    if (!phys_riemann(VL) || !phys_riemann(VR)) { hll_vjp<D>(UL, VL, UR, VR, bF, bUL, bVL, bUR, bVR); return; }
+    const double aL = sound(VL), aR = sound(VR);
+    double ps = 0.0, us = 0.0;
+    int iters = 0;
+    const bool ok = exact_star<D>(VL, VR, aL, aR, ps, us, iters);
+    if (!ok || !finite(ps) || !finite(us) || ps <= 0.0) {
+        hll_vjp<D>(UL, VL, UR, VR, bF, bUL, bVL, bUR, bVR);
+        return;
+    }
+    const bool left = us >= 0.0;
+    const Prim& VK = left ? VL : VR;
+    const double aK = left ? aL : aR;
+    double rho, un, p;
+    exact_sample_nt<D>(VK, aK, ps, us, left ? 0 : 1, rho, un, p);
+    const double ut = ut_of<D>(VK);
+    const Cons Us = cons_nt(rho, un, ut, p, D);
+    const Prim Vs = c2p(Us);
+    if (!phys_riemann(Vs)) { hll_vjp<D>(UL, VL, UR, VR, bF, bUL, bVL, bUR, bVR); return; }
+    // back through the flux and the conversions of the sampled state
+    Cons bUs = czero();
+    Prim bVs = pzero();
+    pflux_vjp<D>(Us, Vs, bF, bUs, bVs);
+    c2p_vjp(Us, bVs, bUs);
+    const Prim Ws = D == kX ? Prim{rho, un, ut, p} : Prim{rho, ut, un, p};   // cons_nt is p2c of it
+    Prim bWs = pzero();
+    p2c_vjp(Ws, bUs, bWs);
+    const double bun = D == kX ? bWs.u : bWs.v, but = D == kX ? bWs.v : bWs.u;
+    double baL = 0.0, baR = 0.0, bps = 0.0, bus = 0.0;
+    Prim& bVK = left ? bVL : bVR;
+    add_ut<D>(bVK, but);
+    exact_sample_vjp<D>(VK, aK, ps, us, left ? 0 : 1, bWs.r, bun, bWs.p, bVK, left ? baL : baR, bps, bus);
+    // u* = (unL + unR + f_R(p*) - f_L(p*)) / 2
+    double fpL, frL, fpKL, faL, fpR, frR, fpKR, faR;
+    exact_pfun_partials(ps, VL.r, VL.p, aL, fpL, frL, fpKL, faL);
+    exact_pfun_partials(ps, VR.r, VR.p, aR, fpR, frR, fpKR, faR);
+    add_un<D>(bVL, 0.5 * bus);
+    add_un<D>(bVR, 0.5 * bus);
+    const double bfR = 0.5 * bus, bfL = -0.5 * bus;
+    bps += bfR * fpR + bfL * fpL;
+    double brL = bfL * frL, bpL = bfL * fpKL, brR = bfR * frR, bpR = bfR * fpKR;
+    baL += bfL * faL;
+    baR += bfR * faR;
+    // p*: f(p*; q) = 0
+    const double lam = bps / (fpL + fpR);
+    brL -= lam * frL; bpL -= lam * fpKL; baL -= lam * faL;
+    brR -= lam * frR; bpR -= lam * fpKR; baR -= lam * faR;
+    add_un<D>(bVL, lam);
+    add_un<D>(bVR, -lam);
+    bVL.r += brL; bVL.p += bpL;
+    bVR.r += brR; bVR.p += bpR;
+    sound_vjp(VL, aL, baL, bVL);
+    sound_vjp(VR, aR, baR, bVR);
+}
+
 template <int S, int D>
 __device__ __forceinline__ void riemann_vjp(const Cons& UL, const Prim& VL, const Cons& UR, const Prim& VR,
                                             const Cons& bF, Cons& bUL, Prim& bVL, Cons& bUR, Prim& bVR) {
     if (S == kHLLC) hllc_vjp<D>(UL, VL, UR, VR, bF, bUL, bVL, bUR, bVR);
+    else if (S == kExact) exact_vjp<D>(UL, VL, UR, VR, bF, bUL, bVL, bUR, bVR);
     else hll_vjp<D>(UL, VL, UR, VR, bF, bUL, bVL, bUR, bVR);
 }
 

@@ -277,13 +277,15 @@ __device__ __forceinline__ Cons cons_nt(double rho, double un, double ut, double
 }
 
 // Sample the solution at xi = 0 on the left (side = 0) or right (side = 1)
-// of the contact (head/riemann.hpp:445-591).
+// of the contact (head/riemann.hpp:445-591): density, normal velocity and
+// pressure there (the tangential velocity is the side's own).
 template <int D>
-__device__ __forceinline__ Cons exact_sample(const Prim& VK, double aK, double p_star, double u_star, int side) {
+__device__ __forceinline__ void exact_sample_nt(const Prim& VK, double aK, double p_star, double u_star, int side,
+                                                double& rho, double& un, double& p) {
     constexpr double g = kGamma;
     const double xi = 0.0;
-    const double rhoK = VK.r, pK = VK.p, unK = un_of<D>(VK), utK = ut_of<D>(VK);
-    double rho = rhoK, un = unK, p = pK;
+    const double rhoK = VK.r, pK = VK.p, unK = un_of<D>(VK);
+    rho = rhoK; un = unK; p = pK;
     if (side == 0) {
         if (p_star > pK) {
             const double p_ratio = p_star / pK;
@@ -333,7 +335,13 @@ __device__ __forceinline__ Cons exact_sample(const Prim& VK, double aK, double p
             }
         }
     }
-    return cons_nt(rho, un, utK, p, D);
+}
+
+template <int D>
+__device__ __forceinline__ Cons exact_sample(const Prim& VK, double aK, double p_star, double u_star, int side) {
+    double rho, un, p;
+    exact_sample_nt<D>(VK, aK, p_star, u_star, side, rho, un, p);
+    return cons_nt(rho, un, ut_of<D>(VK), p, D);
 }
 
 template <int D>
