@@ -198,8 +198,8 @@ int mh_batch_clock(mh_batch b, double* t, double* dt, int* steps, int* status, m
 int mh_batch_info(mh_batch b, int* nblocks, int* nxtiles, int* nytiles);
 
 /* ---- discrete adjoint (no reference counterpart): reverse mode of the
- *      second-order split step at a fixed dt (HLL, HLLC and the exact
- *      solver; FORCE is rejected with MH_ERR_INVALID), with checkpointed
+ *      second-order split step at a fixed dt (all four Riemann solvers;
+ *      the exact solver's p* differentiated implicitly), with checkpointed
  *      forward/backward over N steps. ------------------------------------ */
 /* Workspace for grids of shape desc: checkpoints every checkpoint_every
  * steps for up to max_steps steps. */

@@ -1,5 +1,5 @@
-"""Discrete adjoint of the split MUSCL-Hancock step (HLL, HLLC and the exact
-solver; for the exact solver, the implicit derivative of p* at the converged
+"""Discrete adjoint of the split MUSCL-Hancock step (all four Riemann
+solvers; for the exact solver, the implicit derivative of p* at the converged
 Newton root) and its PyTorch binding.
 
 ``Adjoint`` wraps the C ABI workspace (csrc/adjoint.hip): the vector-Jacobian

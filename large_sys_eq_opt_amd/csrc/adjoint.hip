@@ -261,6 +261,9 @@ int launch_vjp_y(const mh_grid_s* U, const mh_grid_s* lam, mh_grid_s* out, doubl
     } else if (solver == kExact) {
         if (to) hipLaunchKernelGGL((k_vjp_y<kExact, 1>), g, dim3(64), 0, st, u, l, o, dtd, half, seg, nch, ngroups, fold, lo, hi);
         else hipLaunchKernelGGL((k_vjp_y<kExact, 0>), g, dim3(64), 0, st, u, l, o, dtd, half, seg, nch, ngroups, fold, lo, hi);
+    } else if (solver == kForce) {
+        if (to) hipLaunchKernelGGL((k_vjp_y<kForce, 1>), g, dim3(64), 0, st, u, l, o, dtd, half, seg, nch, ngroups, fold, lo, hi);
+        else hipLaunchKernelGGL((k_vjp_y<kForce, 0>), g, dim3(64), 0, st, u, l, o, dtd, half, seg, nch, ngroups, fold, lo, hi);
     } else {
         if (to) hipLaunchKernelGGL((k_vjp_y<kHLL, 1>), g, dim3(64), 0, st, u, l, o, dtd, half, seg, nch, ngroups, fold, lo, hi);
         else hipLaunchKernelGGL((k_vjp_y<kHLL, 0>), g, dim3(64), 0, st, u, l, o, dtd, half, seg, nch, ngroups, fold, lo, hi);
@@ -319,8 +322,7 @@ extern "C" {
 int mh_adjoint_create(const mh_grid_desc* desc, int solver, const mh_bc* bc, int checkpoint_every, int max_steps,
                       mh_adjoint* out) {
     if (!desc || !out || checkpoint_every < 1 || max_steps < 0) return fail(MH_ERR_INVALID, "mh_adjoint_create: bad argument");
-    if (solver != kHLL && solver != kHLLC && solver != kExact)
-        return fail(MH_ERR_INVALID, "mh_adjoint_create: the adjoint covers the HLL, HLLC and exact solvers");
+    if (solver < kHLL || solver > kForce) return fail(MH_ERR_INVALID, "mh_adjoint_create: unknown solver");
     if (bc)
         for (int e = 0; e < 4; ++e)
             if (bc->kind[e] != MH_BC_TRANSMISSIVE && bc->kind[e] != MH_BC_INFLOW)

@@ -1,6 +1,6 @@
 // adjoint_device.h -- hand-written reverse mode (vector-Jacobian products)
 // of the device physics in euler_device.h: conversions, physical flux, sound
-// speed, HLL and HLLC with every branch, the exact solver (implicit
+// speed, HLL, HLLC and FORCE with every branch, the exact solver (implicit
 // derivative of p* and every sampling branch), and the MUSCL-Hancock
 // reconstruction with its limiter and fallbacks.
 //
@@ -409,11 +409,86 @@ __device__ void exact_vjp(const Cons& UL, const Prim& VL, const Cons& UR, const 
     sound_vjp(VR, aR, baR, bVR);
 }
 
+// ---------------------------------------------------------------------------
+// FORCE with local alpha (head/riemann.hpp:284-325): F = (F_LF + F_Ri) / 2,
+// F_LF = (FL + FR)/2 - (alpha/2)(UR - UL), U_Ri = (UL + UR)/2 - (1/(2 alpha))
+// (FR - FL), alpha = fmax(|unL| + aL, |unR| + aR); |x| takes slope -1 only
+// for x < 0.
+// ---------------------------------------------------------------------------
+template <int D>
+__device__ void force_vjp(const Cons& UL, const Prim& VL, const Cons& UR, const Prim& VR, const Cons& bF, Cons& bUL,
+                          Prim& bVL, Cons& bUR, Prim& bVR) {
+    if (!phys_riemann(VL) || !phys_riemann(VR)) { hll_vjp<D>(UL, VL, UR, VR, bF, bUL, bVL, bUR, bVR); return; }
+    const double aL = sound(VL), aR = sound(VR);
+    const double unL = un_of<D>(VL), unR = un_of<D>(VR);
+    const double alpha = lmax(fabs(unL) + aL, fabs(unR) + aR);
+    const Cons FL = pflux<D>(UL, VL), FR = pflux<D>(UR, VR);
+    if (!finite(alpha) || alpha <= 1.0e-14) {
+        const Cons h = {0.5 * bF.r, 0.5 * bF.mu, 0.5 * bF.mv, 0.5 * bF.e};
+        pflux_vjp<D>(UL, VL, h, bUL, bVL);
+        pflux_vjp<D>(UR, VR, h, bUR, bVR);
+        return;
+    }
+    const double ha = 0.5 * alpha, hi = 0.5 * (1.0 / alpha);
+    const double fl[4] = {FL.r, FL.mu, FL.mv, FL.e}, fr[4] = {FR.r, FR.mu, FR.mv, FR.e};
+    const double ul[4] = {UL.r, UL.mu, UL.mv, UL.e}, ur[4] = {UR.r, UR.mu, UR.mv, UR.e};
+    const double bf[4] = {bF.r, bF.mu, bF.mv, bF.e};
+    const Cons Uri = {0.5 * (ul[0] + ur[0]) - hi * (fr[0] - fl[0]), 0.5 * (ul[1] + ur[1]) - hi * (fr[1] - fl[1]),
+                      0.5 * (ul[2] + ur[2]) - hi * (fr[2] - fl[2]), 0.5 * (ul[3] + ur[3]) - hi * (fr[3] - fl[3])};
+    const Prim Vri = c2p(Uri);
+    const bool ri = phys_riemann(Vri);
+    double blf[4], bri[4];
+    for (int c = 0; c < 4; ++c) { blf[c] = ri ? 0.5 * bf[c] : bf[c]; bri[c] = ri ? 0.5 * bf[c] : 0.0; }
+    double bfl[4] = {0.0, 0.0, 0.0, 0.0}, bfr[4] = {0.0, 0.0, 0.0, 0.0}, bul[4], bur[4];
+    double balpha = 0.0;
+    if (ri) {   // F_Ri = pflux(U_Ri, V_Ri)
+        Cons bUri = czero();
+        Prim bVri = pzero();
+        pflux_vjp<D>(Uri, Vri, Cons{bri[0], bri[1], bri[2], bri[3]}, bUri, bVri);
+        c2p_vjp(Uri, bVri, bUri);
+        const double bu[4] = {bUri.r, bUri.mu, bUri.mv, bUri.e};
+        double bhi = 0.0;
+        for (int c = 0; c < 4; ++c) {
+            bhi -= bu[c] * (fr[c] - fl[c]);
+            bfr[c] -= hi * bu[c];
+            bfl[c] += hi * bu[c];
+        }
+        balpha += bhi * (-0.5 / (alpha * alpha));
+        for (int c = 0; c < 4; ++c) { bul[c] = 0.5 * bu[c]; bur[c] = 0.5 * bu[c]; }
+    } else {
+        for (int c = 0; c < 4; ++c) { bul[c] = 0.0; bur[c] = 0.0; }
+    }
+    double bha = 0.0;   // F_LF
+    for (int c = 0; c < 4; ++c) {
+        bfl[c] += 0.5 * blf[c];
+        bfr[c] += 0.5 * blf[c];
+        bha -= blf[c] * (ur[c] - ul[c]);
+        bur[c] -= ha * blf[c];
+        bul[c] += ha * blf[c];
+    }
+    balpha += 0.5 * bha;
+    acc(bUL, bul[0], bul[1], bul[2], bul[3]);
+    acc(bUR, bur[0], bur[1], bur[2], bur[3]);
+    pflux_vjp<D>(UL, VL, Cons{bfl[0], bfl[1], bfl[2], bfl[3]}, bUL, bVL);
+    pflux_vjp<D>(UR, VR, Cons{bfr[0], bfr[1], bfr[2], bfr[3]}, bUR, bVR);
+    double baL = 0.0, baR = 0.0;
+    if ((fabs(unL) + aL) > (fabs(unR) + aR)) {
+        add_un<D>(bVL, unL < 0.0 ? -balpha : balpha);
+        baL += balpha;
+    } else {
+        add_un<D>(bVR, unR < 0.0 ? -balpha : balpha);
+        baR += balpha;
+    }
+    sound_vjp(VL, aL, baL, bVL);
+    sound_vjp(VR, aR, baR, bVR);
+}
+
 template <int S, int D>
 __device__ __forceinline__ void riemann_vjp(const Cons& UL, const Prim& VL, const Cons& UR, const Prim& VR,
                                             const Cons& bF, Cons& bUL, Prim& bVL, Cons& bUR, Prim& bVR) {
     if (S == kHLLC) hllc_vjp<D>(UL, VL, UR, VR, bF, bUL, bVL, bUR, bVR);
     else if (S == kExact) exact_vjp<D>(UL, VL, UR, VR, bF, bUL, bVL, bUR, bVR);
+    else if (S == kForce) force_vjp<D>(UL, VL, UR, VR, bF, bUL, bVL, bUR, bVR);
     else hll_vjp<D>(UL, VL, UR, VR, bF, bUL, bVL, bUR, bVR);
 }
 
