@@ -155,7 +155,8 @@ typedef struct {
     int min_tiles; /* a sweep whose segments give fewer tiles than this gets
                       shorter ones (x: multiples of 60 cells, at least 60;
                       y: at least 4 rows) and, while still below it, one
-                      tile per workgroup; 0 keeps the shape as set */
+                      tile per workgroup; 0 keeps the shape as set
+                      (default 8192) */
 } mh_launch_shape;
 int mh_set_launch_shape(const mh_launch_shape* shape);
 int mh_get_launch_shape(mh_launch_shape* shape);

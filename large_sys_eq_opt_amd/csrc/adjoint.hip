@@ -237,11 +237,15 @@ int launch_transpose(const mh_grid_s* s, mh_grid_s* d, hipStream_t st) {
 }
 
 // Rows per chunk of k_vjp_y: enough chunks for 2048 wavefronts (two per
-// SIMD of 256 CUs), 4..256 rows each.
+// SIMD of 256 CUs), 3..256 rows each.  Three is the fewest the folds allow
+// (ghost rows -2, -1 fold into row 0 and ny, ny+1 into ny-1 within one
+// chunk: the first chunk holds rows -2..0, the last at least 3 rows); small
+// grids are latency-bound, so shorter chunks (3 rows of output for 9 row
+// iterations, against 4 for 10) finish sooner.
 int vjp_seg(const mh_grid_desc& d) {
     const int ngroups = (d.nx + 63) / 64, rows = d.ny + 4;
     const int chunks = (2048 + ngroups - 1) / ngroups;
-    return std::min(256, std::max(4, (rows + chunks - 1) / chunks));
+    return std::min(256, std::max(3, (rows + chunks - 1) / chunks));
 }
 
 // fold < 0: ghost-row adjoints written as they are; else folded by the

@@ -409,7 +409,7 @@ int valid_bc(const mh_bc* b) {
 // wavefront per workgroup and four y tiles per workgroup at the compiler's
 // register choice measured fastest for HLLC at 8192^2 on MI355X; the
 // autotuner can override every field.
-static mh_launch_shape g_shape = {1020, 128, 1, 0, 1, 4, 2048};
+static mh_launch_shape g_shape = {1020, 128, 1, 0, 1, 4, 8192};
 
 template <int S, int L, int WPB>
 void go_x(dim3 g, hipStream_t st, const View& vi, const View& vo, const SweepArgs& a) {
@@ -436,11 +436,13 @@ void go_y_s(int waves, int wpb, dim3 g, hipStream_t st, const View& vi, const Vi
 }
 
 // A grid whose configured segments give fewer than min_tiles tiles (default
-// 2048: two wavefronts on each SIMD of 256 CUs) gets shorter segments, one
-// tile per workgroup: x segments stay a multiple of 60 cells (one 64-lane
-// pass with its 4-cell halo), y chunks at least 4 rows (each chunk reads 4
-// warm-up rows).  Such grids are latency-bound: more, shorter tiles finish
-// sooner although they reread more.
+// 8192: eight wavefronts for each SIMD of 256 CUs, two rounds of the four
+// each holds) gets shorter segments: x segments stay a multiple of 60 cells
+// (one 64-lane pass with its 4-cell halo), y chunks at least 4 rows (each
+// chunk reads 4 warm-up rows).  More, shorter tiles finish sooner although
+// they reread more: at 4096^2 the step fell from 0.65 to 0.58 ms (2048
+// tiles -> 8192), at 8192^2 nothing changes (the default shape gives 8192 y
+// tiles).  A grid left below min_tiles takes one tile per workgroup.
 int seg_x_for(const mh_grid_s* g) {
     const int nx = g->d.nx, ny = g->d.ny, want_tiles = g_shape.min_tiles;
     int seg = g_shape.seg_x > 0 ? g_shape.seg_x : nx;
