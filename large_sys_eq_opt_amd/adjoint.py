@@ -267,10 +267,10 @@ class EulerSolve:
                     g0, gN = Grid2DGPU.from_config(cfg), Grid2DGPU.from_config(cfg)
                     stream = torch.cuda.current_stream()
                     torch.cuda.synchronize()
-                    g0.copy_interior_from_device(U0.data_ptr(), stream)
+                    g0.copy_interior_from_tensor(U0, stream)
                     adj.forward(g0, nsteps, cfl, dt_in, gN, stream)
                     UN = torch.empty_like(U0)
-                    gN.copy_interior_to_device(UN.data_ptr(), stream)
+                    gN.copy_interior_to_tensor(UN, stream)
                     torch.cuda.synchronize()
                     ctx.adj = adj
                     g0.close()
@@ -286,10 +286,10 @@ class EulerSolve:
                     stream = torch.cuda.current_stream()
                     torch.cuda.synchronize()
                     zero_grid(lamN, stream)
-                    lamN.copy_interior_from_device(gUN.data_ptr(), stream)
+                    lamN.copy_interior_from_tensor(gUN, stream)
                     adj.backward(lamN, lam0, stream)
                     g0 = torch.empty_like(gUN)
-                    lam0.copy_interior_to_device(g0.data_ptr(), stream)
+                    lam0.copy_interior_to_tensor(g0, stream)
                     torch.cuda.synchronize()
                     lamN.close()
                     lam0.close()

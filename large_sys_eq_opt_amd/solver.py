@@ -153,10 +153,36 @@ class Grid2DGPU:
         return p.value
 
     def copy_interior_from_device(self, ptr: int, stream=None):
+        """Raw form: `ptr` must address 4*ny*nx float64 device values; prefer
+        copy_interior_from_tensor, which checks the buffer."""
         call("mh_grid_copy_interior_from_device", self._h, C.c_void_p(ptr), _stream(stream))
 
     def copy_interior_to_device(self, ptr: int, stream=None):
+        """Raw form of copy_interior_to_tensor."""
         call("mh_grid_copy_interior_to_device", self._h, C.c_void_p(ptr), _stream(stream))
+
+    def _check_tensor(self, t):
+        """The C ABI takes a bare pointer: check here that the tensor is what
+        it will read or write (a float32 or misshapen tensor would be
+        overrun on the device)."""
+        import torch
+
+        if not isinstance(t, torch.Tensor) or not t.is_cuda:
+            raise TypeError("expected a torch tensor on a GPU")
+        if t.dtype != torch.float64:
+            raise TypeError(f"expected float64, got {t.dtype}")
+        if tuple(t.shape) != (4, self.ny, self.nx) or not t.is_contiguous():
+            raise ValueError(f"expected a contiguous (4, {self.ny}, {self.nx}) tensor, got {tuple(t.shape)}")
+
+    def copy_interior_from_tensor(self, t, stream=None):
+        """Interior <- a contiguous (4, ny, nx) float64 CUDA tensor."""
+        self._check_tensor(t)
+        self.copy_interior_from_device(t.data_ptr(), stream)
+
+    def copy_interior_to_tensor(self, t, stream=None):
+        """A contiguous (4, ny, nx) float64 CUDA tensor <- interior."""
+        self._check_tensor(t)
+        self.copy_interior_to_device(t.data_ptr(), stream)
 
     def swap(self, other: "Grid2DGPU"):
         call("mh_grid_swap", self._h, other._h)

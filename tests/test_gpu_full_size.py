@@ -1,0 +1,188 @@
+"""Size-independent properties at the benchmark's full sizes (8192^2 and
+16384^2 HLLC), where the oracle would take hours: the state stays on the
+device and the checks run there (torch on the copied interior).
+
+  * mirror symmetry of a centred cylindrical blast about both centre lines,
+    to 1e-12 of each field's largest magnitude (the scheme is mirror
+    symmetric up to the order of floating-point sums, as the oracle's own
+    symmetry test finds at small sizes); the initial state is built exactly
+    symmetric from integer cell offsets;
+  * conservation of mass and energy while the waves are far from the
+    boundary (relative 1e-12);
+  * a uniform state stays bit for bit uniform;
+  * the host-synchronous loop (dt from mh_compute_dt each step) and the
+    device-resident loop (dt from the speed maximum fused into the y sweep)
+    give the same dt sequence and the same state bit for bit.
+"""
+from __future__ import annotations
+
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+HLLC = 1
+
+
+def _torch():
+    import torch
+
+    if not torch.cuda.is_available():
+        pytest.fail("no GPU visible to torch")
+    return torch
+
+
+def _blast_state(torch, n, uniform=False):
+    """(4, n, n) conserved interior of the blast on [0,2]^2: inside a radius
+    of 0.4 about the centre rho = p = 1, else rho = 0.125, p = 0.1.  The
+    inside test uses the integer offsets 2i+1-n of the cell centres, so the
+    state is exactly symmetric about both centre lines."""
+    f64 = dict(dtype=torch.float64, device="cuda")
+    c = 2 * torch.arange(n, device="cuda", dtype=torch.int64) + 1 - n
+    inside = (c[None, :] ** 2 + c[:, None] ** 2).double() <= (0.4 * n) ** 2
+    one = torch.ones((), **f64)
+    rho = torch.where(inside, one, 0.125 * one)
+    p = torch.where(inside, one, 0.1 * one)
+    if uniform:
+        rho = torch.ones_like(rho)
+        p = torch.ones_like(p)
+    zero = torch.zeros_like(rho)
+    U = torch.stack([rho, zero, zero.clone(), p / 0.4]).contiguous()
+    assert U.dtype == torch.float64 and U.shape == (4, n, n)
+    return U
+
+
+def _grid_from(hip, torch, U):
+    from large_sys_eq_opt_amd.solver import apply_transmissive_boundary_gpu, synchronize
+
+    n = U.shape[-1]
+    g = hip.Grid2DGPU(n, n, 2, 0.0, 2.0, 0.0, 2.0)
+    torch.cuda.synchronize()
+    g.copy_interior_from_tensor(U)
+    apply_transmissive_boundary_gpu(g)
+    synchronize()
+    return g
+
+
+def _interior(hip, torch, g):
+    from large_sys_eq_opt_amd.solver import synchronize
+
+    out = torch.empty((4, g.ny, g.nx), dtype=torch.float64, device="cuda")
+    synchronize()
+    g.copy_interior_to_tensor(out)
+    synchronize()
+    return out
+
+
+def _check_symmetric(torch, V, tol=1e-12):
+    for axis, odd in ((2, 1), (1, 2)):   # x mirror flips rho*u, y mirror flips rho*v
+        F = torch.flip(V, dims=[axis])
+        for c in range(4):
+            sign = -1.0 if c == odd else 1.0
+            scale = V[c].abs().max().item()
+            gap = (V[c] - sign * F[c]).abs().max().item()
+            assert gap <= tol * max(scale, 1e-300), (axis, c, gap, scale)
+
+
+@pytest.mark.parametrize("n, steps", [(8192, 10), (16384, 3)])
+def test_full_size_blast_symmetry_and_conservation(hip, n, steps):
+    torch = _torch()
+    U0 = _blast_state(torch, n)
+    mass0, energy0 = U0[0].sum().item(), U0[3].sum().item()
+    g = _grid_from(hip, torch, U0)
+    del U0
+    st = hip.Stepper(g, HLLC, 0.8, 0.0, None, log_capacity=steps)
+    st.advance(steps)
+    st.finish()
+    _, nsteps, dts = st.clock()
+    st.close()
+    assert nsteps == steps and (dts > 0).all()
+    V = _interior(hip, torch, g)
+    assert torch.isfinite(V).all().item()
+    _check_symmetric(torch, V)
+    # waves travel about (steps * 0.8 * dx) << distance to the boundary
+    assert abs(V[0].sum().item() - mass0) <= 1e-12 * mass0
+    assert abs(V[3].sum().item() - energy0) <= 1e-12 * energy0
+    g.close()
+
+
+def test_full_size_uniform_state_stays_uniform(hip):
+    torch = _torch()
+    n = 8192
+    U0 = _blast_state(torch, n, uniform=True)
+    g = _grid_from(hip, torch, U0)
+    st = hip.Stepper(g, HLLC, 0.8, 0.0, None, log_capacity=4)
+    st.advance(4)
+    st.finish()
+    st.close()
+    V = _interior(hip, torch, g)
+    assert torch.equal(V, U0)
+    g.close()
+
+
+def test_full_size_host_and_device_loops_agree(hip):
+    from large_sys_eq_opt_amd import advance_second_order_gpu, compute_dt_gpu
+
+    torch = _torch()
+    n, steps = 8192, 4
+    U0 = _blast_state(torch, n)
+    a = _grid_from(hip, torch, U0)
+    st = hip.Stepper(a, HLLC, 0.8, 0.0, None, log_capacity=steps)
+    st.advance(steps)
+    st.finish()
+    _, _, dts_dev = st.clock()
+    st.close()
+    Va = _interior(hip, torch, a)
+    a.close()
+    b = _grid_from(hip, torch, U0)
+    tmp, nxt = (hip.Grid2DGPU(n, n, 2, 0.0, 2.0, 0.0, 2.0) for _ in range(2))
+    dts_host = []
+    for _ in range(steps):
+        dt = compute_dt_gpu(b, None, 0.8)
+        dts_host.append(dt)
+        advance_second_order_gpu(b, tmp, nxt, None, dt, HLLC)
+        b.swap(nxt)
+    Vb = _interior(hip, torch, b)
+    assert list(dts_dev) == dts_host
+    assert torch.equal(Va, Vb)
+    for g in (b, tmp, nxt):
+        g.close()
+
+
+def test_full_size_host_round_trip(hip):
+    """AoS upload and download of an 8192^2 state (2.15 GB each way) and the
+    device interior copies agree cell for cell."""
+    import numpy as np
+
+    torch = _torch()
+    n = 8192
+    rng = np.random.default_rng(0)
+    host = rng.standard_normal((n + 4, n + 4, 4))
+    g = hip.Grid2DGPU(n, n, 2, 0.0, 2.0, 0.0, 2.0)
+    g.upload_from_aos(host)
+    back = g.download_to_aos()
+    assert np.array_equal(back, host)
+    V = _interior(hip, torch, g).cpu().numpy()
+    assert np.array_equal(V, np.moveaxis(host[2:-2, 2:-2], -1, 0))
+    g.close()
+
+
+def test_tensor_copies_check_dtype_and_shape(hip):
+    """The tensor forms of the interior copies refuse, before any device
+    copy, a tensor the C ABI would overrun or underfill."""
+    torch = _torch()
+    g = hip.Grid2DGPU(40, 30, 2, 0.0, 1.0, 0.0, 1.0)
+    for bad in (torch.zeros((4, 30, 40), dtype=torch.float32, device="cuda"),
+                torch.zeros((4, 30, 39), dtype=torch.float64, device="cuda"),
+                torch.zeros((4, 40, 30), dtype=torch.float64, device="cuda").transpose(1, 2),
+                torch.zeros((4, 30, 40), dtype=torch.float64)):
+        with pytest.raises((TypeError, ValueError)):
+            g.copy_interior_from_tensor(bad)
+        with pytest.raises((TypeError, ValueError)):
+            g.copy_interior_to_tensor(bad)
+    ok = torch.arange(4 * 30 * 40, dtype=torch.float64, device="cuda").reshape(4, 30, 40)
+    g.copy_interior_from_tensor(ok)
+    back = torch.empty_like(ok)
+    g.copy_interior_to_tensor(back)
+    torch.cuda.synchronize()
+    assert torch.equal(back, ok)
+    g.close()
