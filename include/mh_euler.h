@@ -15,6 +15,13 @@
  * means Conserved{rho,rhou,rhov,E} per cell (head/types.hpp:13-23), "SoA"
  * means four planes rho, rhou, rhov, E (head/gpu/grid_gpu.cuh:43-46).
  * On the device a grid is four FP64 planes with a padded row pitch.
+ *
+ * Concurrency: a grid carries its own device scratch (a max-speed slot and
+ * the y sweep's list of tiles to redo with IEEE division), so calls that
+ * write the same grid, or compute dt from it, must not overlap (order them
+ * on one stream or with events); calls on different grids may run on
+ * different streams at once.  The sweeps also write the column left of the
+ * two left ghost columns (row padding that no call reads).
  */
 #ifndef MH_EULER_H
 #define MH_EULER_H
