@@ -51,6 +51,19 @@ def interior(U):
     return U[2:-2, 2:-2]
 
 
+def same_bits(a, b) -> bool:
+    """Bit-for-bit equality of two float64 arrays (so +0 and -0 differ).
+    NaN payloads are not portable between the GPU and the host libm, so
+    NaN entries only have to be NaN on both sides."""
+    a, b = np.ascontiguousarray(a, dtype=np.float64), np.ascontiguousarray(b, dtype=np.float64)
+    if a.shape != b.shape:
+        return False
+    na, nb = np.isnan(a), np.isnan(b)
+    if not np.array_equal(na, nb):
+        return False
+    return bool(np.array_equal(a[~na].view(np.uint64), b[~nb].view(np.uint64)))
+
+
 def rel_linf(a, b):
     """max |a-b| / max |b| per conserved component, maximised."""
     a, b = interior(a), interior(b)

@@ -2,6 +2,8 @@
 from __future__ import annotations
 
 import numpy as np
+
+from conftest import same_bits
 import pytest
 
 from large_sys_eq_opt_amd import cases
@@ -36,7 +38,7 @@ def test_post_shock_state():
 def test_host_initial_state_matches_oracle(oracle, name):
     cfg = cases.get_case_config(name)
     kind, params = cases.case_params(name)
-    assert np.array_equal(cases.initialise_grid(name, cfg), oracle.init(kind, params.array(), cfg.nx, cfg.ny,
+    assert same_bits(cases.initialise_grid(name, cfg), oracle.init(kind, params.array(), cfg.nx, cfg.ny,
                                                                            cfg.bounds))
 
 
@@ -44,7 +46,7 @@ def test_reference_initial_grid(reference):
     for case_id, name in ((0, "shock_bubble"), (1, "blast_wave")):
         U, _ = reference.init_case(case_id, 1)
         mine, _ = cases.make_n_grid(name, 1)
-        assert np.array_equal(U, mine)
+        assert same_bits(U, mine)
 
 
 def test_solver_names():
@@ -66,7 +68,7 @@ def test_snapshot_csv_format_and_roundtrip(tmp_path):
     assert len(lines) == 1 + 24
     exact = str(tmp_path / "e.csv")
     write_grid_csv(U, cfg, exact, precision=17)
-    assert np.array_equal(read_grid_csv(exact, 6, 4), U[2:-2, 2:-2])
+    assert same_bits(read_grid_csv(exact, 6, 4), U[2:-2, 2:-2])
     assert snapshot_name("gpu", "blast_wave", "hllc", 2, 3) == "outputs/gpu_blast_wave_hllc_n2_snapshot_3.csv"
 
 

@@ -13,7 +13,7 @@ import ctypes as C
 import numpy as np
 import pytest
 
-from conftest import interior, random_states, rel_linf
+from conftest import interior, random_states, rel_linf, same_bits
 
 pytestmark = pytest.mark.gpu
 
@@ -72,7 +72,7 @@ def test_riemann_flux_parity(hip, oracle, solver, direction):
     Fo = oracle.riemann_flux(UL, UR, direction, solver)
     Fg, _ = gpu_flux(hip, UL, UR, direction, solver)
     if BITWISE[solver]:
-        assert np.array_equal(Fo, Fg, equal_nan=True), np.nanmax(np.abs(Fo - Fg))
+        assert same_bits(Fo, Fg), np.nanmax(np.abs(Fo - Fg))
     else:
         ok = np.isfinite(Fo)
         assert np.array_equal(np.isfinite(Fg), ok)
@@ -89,8 +89,8 @@ def test_reconstruction_parity(hip, oracle, direction):
     for dtd in (0.01, 0.3, 2.0):
         Lo, Ro = oracle.reconstruct(Um, Uc, Up, dtd, direction)
         Lg, Rg = gpu_recon(hip, Um, Uc, Up, dtd, direction)
-        assert np.array_equal(Lo, Lg, equal_nan=True)
-        assert np.array_equal(Ro, Rg, equal_nan=True)
+        assert same_bits(Lo, Lg)
+        assert same_bits(Ro, Rg)
 
 
 def random_field(oracle, rng, nx, ny):
@@ -123,7 +123,7 @@ def test_sweep_parity(hip, oracle, solver, direction, shape):
     sweep_gpu(a, b, dt, direction, solver)
     out = b.download_to_aos()
     if BITWISE[solver]:
-        assert np.array_equal(interior(out), interior(ref))
+        assert same_bits(interior(out), interior(ref))
     else:
         assert rel_linf(out, ref) <= EXACT_STATE_TOL
 
@@ -139,8 +139,8 @@ def test_initial_conditions_bitwise(hip, oracle):
         dev = g.download_to_aos()
         orc = oracle.init(kind, params.array(), cfg.nx, cfg.ny, cfg.bounds)
         host = cases.initialise_grid(name, cfg)
-        assert np.array_equal(dev, orc), name
-        assert np.array_equal(host, orc), name
+        assert same_bits(dev, orc), name
+        assert same_bits(host, orc), name
 
 
 def _run_both(hip, oracle, case, solver, scale=1, mode="host", nsnap=0, max_steps=None):
@@ -161,8 +161,8 @@ def _run_both(hip, oracle, case, solver, scale=1, mode="host", nsnap=0, max_step
 def _assert_runs_match(o, g, solver):
     assert g["steps"] == o["steps"]
     if BITWISE[solver]:
-        assert np.array_equal(g["dt"], o["dt"]), "dt sequence differs"
-        assert np.array_equal(interior(g["U"]), interior(o["U"]))
+        assert same_bits(g["dt"], o["dt"]), "dt sequence differs"
+        assert same_bits(interior(g["U"]), interior(o["U"]))
     else:
         assert np.max(np.abs(g["dt"] - o["dt"]) / o["dt"]) <= EXACT_DT_TOL
         assert rel_linf(g["U"], o["U"]) <= EXACT_STATE_TOL
@@ -231,8 +231,8 @@ def test_launch_shapes_parity(hip, oracle, solver, k):
         set_launch_shape(**saved)
     assert n == steps
     if BITWISE[solver]:
-        assert np.array_equal(dts, o["dt"])
-        assert np.array_equal(interior(out), interior(o["U"]))
+        assert same_bits(dts, o["dt"])
+        assert same_bits(interior(out), interior(o["U"]))
     else:
         assert np.max(np.abs(dts - o["dt"]) / o["dt"]) <= EXACT_DT_TOL
         assert rel_linf(out, o["U"]) <= EXACT_STATE_TOL

@@ -18,7 +18,7 @@ import os
 import numpy as np
 import pytest
 
-from conftest import interior, random_states, smooth_state
+from conftest import interior, random_states, same_bits, smooth_state
 
 GOLD = np.load(os.path.join(os.path.dirname(__file__), "golden", "golden.npz"))
 RUN_NAMES = ["cylblast24", "sbnd48", "refblast20"]
@@ -29,22 +29,22 @@ RUN_NAMES = ["cylblast24", "sbnd48", "refblast20"]
 @pytest.mark.parametrize("direction", [0, 1])
 def test_flux_matches_reference_golden(oracle, solver, direction):
     F = oracle.riemann_flux(GOLD["flux_UL"], GOLD["flux_UR"], direction, solver)
-    assert np.array_equal(F, GOLD[f"flux_s{solver}_d{direction}"], equal_nan=True)
+    assert same_bits(F, GOLD[f"flux_s{solver}_d{direction}"])
 
 
 @pytest.mark.parametrize("direction", [0, 1])
 def test_reconstruction_matches_reference_golden(oracle, direction):
     for k, dtd in enumerate(GOLD["rec_dtd"]):
         L, R = oracle.reconstruct(GOLD["rec_Um"], GOLD["rec_Uc"], GOLD["rec_Up"], float(dtd), direction)
-        assert np.array_equal(L, GOLD[f"rec_L_d{direction}_k{k}"])
-        assert np.array_equal(R, GOLD[f"rec_R_d{direction}_k{k}"])
+        assert same_bits(L, GOLD[f"rec_L_d{direction}_k{k}"])
+        assert same_bits(R, GOLD[f"rec_R_d{direction}_k{k}"])
 
 
 def test_exact_star_matches_reference_golden(oracle):
     ps, us, _, ok = oracle.exact_star(GOLD["star_VL"], GOLD["star_VR"], 0)
     assert np.array_equal(ok.astype(np.float64), GOLD["star_ok"])
     m = ok.astype(bool)
-    assert np.array_equal(ps[m], GOLD["star_p"][m]) and np.array_equal(us[m], GOLD["star_u"][m])
+    assert same_bits(ps[m], GOLD["star_p"][m]) and same_bits(us[m], GOLD["star_u"][m])
 
 
 @pytest.mark.parametrize("solver", [0, 1, 2, 3])
@@ -56,8 +56,8 @@ def test_small_runs_match_reference_golden(oracle, solver, k):
     dx, dy = (bounds[1] - bounds[0]) / nx, (bounds[3] - bounds[2]) / ny
     infl = GOLD[f"run_{name}_inflow"] if inflow else None
     r = oracle.run(GOLD[f"run_{name}_U0"], dx, dy, solver, cfl, max_steps=steps, inflow_left=infl)
-    assert np.array_equal(r["dt"], GOLD[f"run_{name}_s{solver}_dt"])
-    assert np.array_equal(interior(r["U"]), GOLD[f"run_{name}_s{solver}_U"])
+    assert same_bits(r["dt"], GOLD[f"run_{name}_s{solver}_dt"])
+    assert same_bits(interior(r["U"]), GOLD[f"run_{name}_s{solver}_U"])
 
 
 def test_oracle_matches_compiled_reference_directly(oracle, reference):
@@ -66,10 +66,10 @@ def test_oracle_matches_compiled_reference_directly(oracle, reference):
     UR = oracle.prim_to_cons(random_states(rng, 3000))
     for s in range(4):
         for d in range(2):
-            assert np.array_equal(oracle.riemann_flux(UL, UR, d, s), reference.riemann_flux(UL, UR, d, s))
+            assert same_bits(oracle.riemann_flux(UL, UR, d, s), reference.riemann_flux(UL, UR, d, s))
     for case in (0, 1):   # the reference's own initial grids
         U, c = reference.init_case(case, 1)
-        assert np.array_equal(oracle.init(case, None, c["nx"], c["ny"], c["bounds"]), U)
+        assert same_bits(oracle.init(case, None, c["nx"], c["ny"], c["bounds"]), U)
 
 
 def test_full_loop_matches_compiled_reference(oracle, reference):
@@ -79,12 +79,12 @@ def test_full_loop_matches_compiled_reference(oracle, reference):
     for s in range(4):
         a = reference.run(U, c["bounds"], s, c["cfl"], max_steps=12)
         b = oracle.run(U, dx, dy, s, c["cfl"], max_steps=12)
-        assert np.array_equal(a["dt"], b["dt"]) and np.array_equal(interior(a["U"]), interior(b["U"]))
+        assert same_bits(a["dt"], b["dt"]) and same_bits(interior(a["U"]), interior(b["U"]))
     # first-order path (src/cpu/solver_cpu.cpp:315-369)
     for s in range(4):
         a = reference.run(U, c["bounds"], s, c["cfl"], max_steps=6, order=1)
         b = oracle.run(U, dx, dy, s, c["cfl"], max_steps=6, order=1)
-        assert np.array_equal(a["dt"], b["dt"]) and np.array_equal(interior(a["U"]), interior(b["U"]))
+        assert same_bits(a["dt"], b["dt"]) and same_bits(interior(a["U"]), interior(b["U"]))
 
 
 # ---------------------------------------------------------------- recorded known answers
@@ -172,7 +172,7 @@ def test_xy_rotation_symmetry_bitwise(oracle, solver):
     UL, UR = oracle.prim_to_cons(VL), oracle.prim_to_cons(VR)
     Fx = oracle.riemann_flux(UL, UR, 0, solver)
     Fy = oracle.riemann_flux(UL[:, swap], UR[:, swap], 1, solver)
-    assert np.array_equal(Fx, Fy[:, swap])
+    assert same_bits(Fx, Fy[:, swap])
 
 
 def test_uniform_state_is_preserved_bitwise(oracle):
@@ -181,7 +181,7 @@ def test_uniform_state_is_preserved_bitwise(oracle):
     U[...] = oracle.prim_to_cons(np.array([[1.3, 0.4, -0.2, 2.0]]))[0]
     for s in range(4):
         r = oracle.run(U, 0.05, 0.05, s, 0.8, max_steps=5)
-        assert np.array_equal(r["U"], U)
+        assert same_bits(r["U"], U)
 
 
 def test_mass_and_energy_conserved_before_waves_reach_boundary(oracle):
@@ -231,7 +231,7 @@ def test_dual_tangent_matches_central_differences(oracle):
             a, b, q = rng.uniform(0.5, 2.0, 3)
             dU[2:-2, 2:-2, c] = w * np.sin(2 * np.pi * (a * x + b * y) + q)
         Ut, T = oracle.run_tangent(U0, dU, dx, dy, solver, dts)
-        assert np.array_equal(interior(Ut), interior(r["U"]))
+        assert same_bits(interior(Ut), interior(r["U"]))
         h = 1e-6
         P = oracle.run(U0 + h * dU, dx, dy, solver, 0.8, dt_in=dts, max_steps=len(dts))["U"]
         M = oracle.run(U0 - h * dU, dx, dy, solver, 0.8, dt_in=dts, max_steps=len(dts))["U"]
