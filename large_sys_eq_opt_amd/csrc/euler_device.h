@@ -459,7 +459,15 @@ __device__ __forceinline__ Cons update(const Cons& U, double dtd, const Cons& Fm
 template <bool F = false>
 __device__ __forceinline__ double signal_speed(const Cons& U) {
     const Prim V = c2p<F>(U);
-    const double a = sound<F>(V);
+    double a;
+    if constexpr (F) {
+        // sqrt_nr(+-0) is NaN (rsq(0) = inf, times 0), sqrt(+-0) is +-0: a
+        // state with p exactly 0 must still enter the maximum with |u|, |v|
+        const double s = dv<true>(kGamma * V.p, V.r);
+        a = (s == 0.0) ? s : sqrt_nr(s);
+    } else {
+        a = sound<false>(V);
+    }
     return smax(fabs(V.u) + a, fabs(V.v) + a);
 }
 

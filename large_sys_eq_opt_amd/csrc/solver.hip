@@ -376,6 +376,24 @@ __global__ __launch_bounds__(256) void k_recon(int n, const double* Um, const do
     R[4 * k] = r.U.r; R[4 * k + 1] = r.U.mu; R[4 * k + 2] = r.U.mv; R[4 * k + 3] = r.U.e;
 }
 
+// Pointwise fast-path arithmetic for the tests: q = a / b, r = sqrt(a) and
+// s = signal speed of U with the unscaled sequences (fast) or the compiler's.
+__global__ __launch_bounds__(256) void k_fast_ops(int n, const double* a, const double* b, const double* U, int fast,
+                                                 double* q, double* r, double* s) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const Cons u = {U[4 * k], U[4 * k + 1], U[4 * k + 2], U[4 * k + 3]};
+    if (fast) {
+        q[k] = div_nr(a[k], b[k]);
+        r[k] = sqrt_nr(a[k]);
+        s[k] = signal_speed<true>(u);
+    } else {
+        q[k] = a[k] / b[k];
+        r[k] = sqrt(a[k]);
+        s[k] = signal_speed<false>(u);
+    }
+}
+
 // Plain streaming copy (calibrates traffic counters; measured copy peak).
 __global__ __launch_bounds__(256) void k_copy(const double2* __restrict__ src, double2* __restrict__ dst, int64_t n2) {
     for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n2; k += (int64_t)gridDim.x * blockDim.x)
@@ -915,6 +933,32 @@ struct DevBuf {
     void* p = nullptr;
     ~DevBuf() { if (p) (void)hipFree(p); }
 };
+
+int mh_fast_ops(int n, const double* a, const double* b, const double* U, int fast, double* q, double* r, double* s) {
+    if (n < 0 || (n > 0 && (!a || !b || !U || !q || !r || !s)) || (fast != 0 && fast != 1))
+        return fail(MH_ERR_INVALID, "mh_fast_ops: bad argument");
+    if (n == 0) return MH_OK;
+    const size_t nb = sizeof(double) * (size_t)n;
+    DevBuf buf;
+    int rc = check_hip(hipMalloc(&buf.p, 10 * nb), "hipMalloc");
+    if (rc) return rc;
+    double* da = (double*)buf.p;
+    double* db = da + n;
+    double* dU = db + n;
+    double* dq = dU + 4 * (size_t)n;
+    double* dr = dq + n;
+    double* ds = dr + n;
+    rc = check_hip(hipMemcpy(da, a, nb, hipMemcpyHostToDevice), "hipMemcpy");
+    if (!rc) rc = check_hip(hipMemcpy(db, b, nb, hipMemcpyHostToDevice), "hipMemcpy");
+    if (!rc) rc = check_hip(hipMemcpy(dU, U, 4 * nb, hipMemcpyHostToDevice), "hipMemcpy");
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_fast_ops, dim3((n + 255) / 256), dim3(256), 0, 0, n, da, db, dU, fast, dq, dr, ds);
+    rc = check_hip(hipGetLastError(), "fast ops launch");
+    if (!rc) rc = check_hip(hipMemcpy(q, dq, nb, hipMemcpyDeviceToHost), "hipMemcpy");
+    if (!rc) rc = check_hip(hipMemcpy(r, dr, nb, hipMemcpyDeviceToHost), "hipMemcpy");
+    if (!rc) rc = check_hip(hipMemcpy(s, ds, nb, hipMemcpyDeviceToHost), "hipMemcpy");
+    return rc;
+}
 
 int mh_riemann_flux(int n, const double* UL, const double* UR, int dir, int solver, double* F, int* iters) {
     if (n < 0 || (n > 0 && (!UL || !UR || !F))) return fail(MH_ERR_INVALID, "mh_riemann_flux: bad argument");

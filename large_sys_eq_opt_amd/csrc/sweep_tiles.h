@@ -185,7 +185,8 @@ __device__ __forceinline__ bool y_tile_pass(const View& in, const View& out, int
             // about 2^-150..2^400 (differences of in-range values cannot fall
             // below their unit in the last place), so its signal speed's
             // 1/rho, gamma*p/rho and square root stay exact with the fast
-            // sequences; a NaN speed never enters the maximum.
+            // sequences (signal_speed<true> takes the square root of zero
+            // apart); a NaN speed never enters the maximum.
             if (speed) m = smax(m, signal_speed<F>(Unew));
             if (ghosts) {
                 if (i == 0) {

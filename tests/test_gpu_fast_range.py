@@ -173,3 +173,67 @@ def test_batch_out_of_range_instance(hip, oracle, schedule):
         o = oracle.run(U0, 2.0 / side, 2.0 / side, 1, 0.8, max_steps=steps)
         assert same_bits(interior(b.download(k)), interior(o["U"])), (k, s)
     b.close()
+
+
+def _fast_ops(a, b, U, fast):
+    import ctypes as C
+
+    from large_sys_eq_opt_amd._lib import call
+
+    a, b, U = (np.ascontiguousarray(x, dtype=np.float64) for x in (a, b, U))
+    n = len(a)
+    q, r, s = (np.empty(n) for _ in range(3))
+    ptr = lambda x: x.ctypes.data_as(C.c_void_p)   # noqa: E731
+    call("mh_fast_ops", n, ptr(a), ptr(b), ptr(U), int(fast), ptr(q), ptr(r), ptr(s))
+    return q, r, s
+
+
+def _random_doubles(rng, n, lo, hi):
+    """Random signs, mantissas in [1, 2) and exponents uniform in [lo, hi]."""
+    return rng.choice([-1.0, 1.0], n) * np.ldexp(rng.uniform(1.0, 2.0, n), rng.integers(lo, hi + 1, n))
+
+
+def test_unscaled_division_and_sqrt_match_ieee(hip):
+    """div_nr and sqrt_nr against the compiler's division and square root on
+    the GPU, bit for bit: 2^20 operand pairs with exponents in [-300, 300]
+    (the fast range keeps every operand within 2^+-310 and exponent gaps
+    below 500, so this covers it with margin) and square roots of positive
+    operands in [2^-300, 2^300].  Zero, infinite and NaN operands of the
+    division match as well (v_div_fixup is kept); the square root of zero
+    does not (NaN), which is why the signal speed takes it apart."""
+    rng = np.random.default_rng(2024)
+    n = 1 << 20
+    a = _random_doubles(rng, n, -300, 300)
+    b = _random_doubles(rng, n, -300, 300)
+    U = np.tile([1.0, 0.5, -0.25, 3.0], (n, 1))
+    qf, rf, _ = _fast_ops(a, b, U, 1)
+    qi, ri, _ = _fast_ops(a, b, U, 0)
+    assert same_bits(qf, qi)
+    pos = a > 0
+    assert same_bits(rf[pos], ri[pos])
+    special = np.array([0.0, -0.0, np.inf, -np.inf, np.nan, 1.0, -3.5])
+    A, B = (x.ravel() for x in np.meshgrid(special, special))
+    qf, rf, _ = _fast_ops(A, B, np.tile([1.0, 0.0, 0.0, 2.5], (len(A), 1)), 1)
+    qi, ri, _ = _fast_ops(A, B, np.tile([1.0, 0.0, 0.0, 2.5], (len(A), 1)), 0)
+    assert same_bits(qf, qi)
+    assert np.isnan(rf[A == 0.0]).all() and (ri[A == 0.0] == 0.0).all()
+
+
+def test_fast_signal_speed_matches_ieee(hip):
+    """The y sweep's fused signal speed with the fast sequences equals the
+    compiler's, bit for bit, for random in-range states and for states with
+    p exactly 0 (rho = 1, rho*u = 4, E = 8), where the square root of zero
+    needs its own case."""
+    rng = np.random.default_rng(7)
+    n = 1 << 18
+    rho = np.ldexp(rng.uniform(1.0, 2.0, n), rng.integers(-20, 20, n))
+    u, v = rng.uniform(-3.0, 3.0, n), rng.uniform(-3.0, 3.0, n)
+    p = np.ldexp(rng.uniform(1.0, 2.0, n), rng.integers(-20, 20, n))
+    U = np.stack([rho, rho * u, rho * v, p / 0.4 + 0.5 * rho * (u * u + v * v)], axis=1)
+    U[:16] = [1.0, 4.0, 0.0, 8.0]
+    U[16:32] = [2.0, 0.0, -6.0, 9.0]
+    ones = np.ones(n)
+    _, _, sf = _fast_ops(ones, ones, U, 1)
+    _, _, si = _fast_ops(ones, ones, U, 0)
+    assert (si[:16] == 4.0).all() and (si[16:32] == 3.0).all()
+    assert same_bits(sf, si)
