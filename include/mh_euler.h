@@ -252,7 +252,9 @@ int mh_riemann_flux(int n, const double* UL, const double* UR, int dir, int solv
  * that pin it to the compiler's (host buffers of n values; U is n AoS
  * states).  fast = 1: q = a/b and r = sqrt(a) by the unscaled Newton
  * sequences, s = the signal speed max(|u|+c, |v|+c) as the y sweep's fused
- * dt reduction forms it; fast = 0: the compiler's division and square root. */
+ * dt reduction forms it; fast = 2: the same with the division's final
+ * special-case fixup left out (as the sweeps do where operands are normal);
+ * fast = 0: the compiler's division and square root. */
 int mh_fast_ops(int n, const double* a, const double* b, const double* U, int fast, double* q, double* r, double* s);
 /* reconstruct_cell_muscl_hancock (src/cpu/solver_cpu.cpp:68-99) for n cells:
  * evolved left/right edge states from (Um, Uc, Up) and dt/d. */

@@ -45,16 +45,26 @@ __device__ __forceinline__ bool finite(double x) { return __builtin_isfinite(x);
 // inside the normal range; the sweeps keep their results (F = true) only
 // for tiles whose states lie in the fast range (range_key), where every
 // operand does.
-__device__ __forceinline__ double div_nr(double a, double b) {
+// div_nr_nf is div_nr without v_div_fixup, which changes nothing unless an
+// operand is zero, infinite, NaN or denormal.  It serves only quotients
+// whose operands are normal and nonzero wherever the result is kept: 1/rho
+// of a converted state, p/(g-1) of an edge state that passed the floor
+// check, g p/rho and E/rho of states that passed the physicality check,
+// p/(rho (S - u)) of the HLLC star state (a zero denominator there has
+// returned before).  Quotients whose numerator may be +-0 (HLL, S*, rho*)
+// keep the fixup, which gives the zero its IEEE sign.
+__device__ __forceinline__ double div_nr_nf(double a, double b) {
     double r = __builtin_amdgcn_rcp(b);
     double e = __builtin_fma(-b, r, 1.0);
     r = __builtin_fma(r, e, r);
     e = __builtin_fma(-b, r, 1.0);
     r = __builtin_fma(r, e, r);
-    double q = a * r;
+    const double q = a * r;
     const double t = __builtin_fma(-b, q, a);
-    q = __builtin_fma(t, r, q);
-    return __builtin_amdgcn_div_fixup(q, b, a);
+    return __builtin_fma(t, r, q);
+}
+__device__ __forceinline__ double div_nr(double a, double b) {
+    return __builtin_amdgcn_div_fixup(div_nr_nf(a, b), b, a);
 }
 __device__ __forceinline__ double sqrt_nr(double x) {
     const double y = __builtin_amdgcn_rsq(x);
@@ -69,6 +79,10 @@ __device__ __forceinline__ double sqrt_nr(double x) {
 }
 template <bool F> __device__ __forceinline__ double dv(double a, double b) {
     if constexpr (F) return div_nr(a, b);
+    else return a / b;
+}
+template <bool F> __device__ __forceinline__ double dvn(double a, double b) {
+    if constexpr (F) return div_nr_nf(a, b);
     else return a / b;
 }
 template <bool F> __device__ __forceinline__ double sq(double x) {
@@ -88,7 +102,7 @@ __device__ __forceinline__ double smax(double a, double b) { return (a < b) ? b 
 // head/physics.hpp:13-20
 template <bool F = false>
 __device__ __forceinline__ Prim c2p(const Cons& U) {
-    const double inv_rho = dv<F>(1.0, U.r);
+    const double inv_rho = dvn<F>(1.0, U.r);
     const double u = U.mu * inv_rho;
     const double v = U.mv * inv_rho;
     const double kinetic = 0.5 * U.r * (u * u + v * v);
@@ -99,11 +113,11 @@ __device__ __forceinline__ Prim c2p(const Cons& U) {
 template <bool F = false>
 __device__ __forceinline__ Cons p2c(const Prim& V) {
     const double kinetic = 0.5 * V.r * (V.u * V.u + V.v * V.v);
-    return {V.r, V.r * V.u, V.r * V.v, dv<F>(V.p, kGm1) + kinetic};
+    return {V.r, V.r * V.u, V.r * V.v, dvn<F>(V.p, kGm1) + kinetic};
 }
 
 template <bool F = false>
-__device__ __forceinline__ double sound(const Prim& V) { return sq<F>(dv<F>(kGamma * V.p, V.r)); }  // :39-41
+__device__ __forceinline__ double sound(const Prim& V) { return sq<F>(dvn<F>(kGamma * V.p, V.r)); }  // :39-41
 
 // Physical flux of U along D given V = c2p(U) (head/physics.hpp:56-74).
 template <int D> __device__ __forceinline__ Cons pflux(const Cons& U, const Prim& V) {
@@ -160,8 +174,8 @@ __device__ __forceinline__ Cons hllc_star(const Cons& U, const Prim& V, double S
     if (fabs(denom) < 1.0e-14) return U;
     const double rho_star = dv<F>(rho * (S - un), denom);
     if (!finite(rho_star) || rho_star <= 0.0) return U;
-    const double E_per_rho = dv<F>(U.e, rho);
-    const double E_star = rho_star * (E_per_rho + (Sstar - un) * (Sstar + dv<F>(p, rho * (S - un))));
+    const double E_per_rho = dvn<F>(U.e, rho);
+    const double E_star = rho_star * (E_per_rho + (Sstar - un) * (Sstar + dvn<F>(p, rho * (S - un))));
     if (!finite(E_star)) return U;
     if (D == kX) return {rho_star, rho_star * Sstar, rho_star * ut, E_star};
     return {rho_star, rho_star * ut, rho_star * Sstar, E_star};
@@ -463,7 +477,7 @@ __device__ __forceinline__ double signal_speed(const Cons& U) {
     if constexpr (F) {
         // sqrt_nr(+-0) is NaN (rsq(0) = inf, times 0), sqrt(+-0) is +-0: a
         // state with p exactly 0 must still enter the maximum with |u|, |v|
-        const double s = dv<true>(kGamma * V.p, V.r);
+        const double s = dvn<true>(kGamma * V.p, V.r);
         a = (s == 0.0) ? s : sqrt_nr(s);
     } else {
         a = sound<false>(V);

@@ -377,14 +377,15 @@ __global__ __launch_bounds__(256) void k_recon(int n, const double* Um, const do
 }
 
 // Pointwise fast-path arithmetic for the tests: q = a / b, r = sqrt(a) and
-// s = signal speed of U with the unscaled sequences (fast) or the compiler's.
+// s = signal speed of U with the unscaled sequences (fast = 1; fast = 2
+// divides without the fixup) or the compiler's (fast = 0).
 __global__ __launch_bounds__(256) void k_fast_ops(int n, const double* a, const double* b, const double* U, int fast,
                                                  double* q, double* r, double* s) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
     const Cons u = {U[4 * k], U[4 * k + 1], U[4 * k + 2], U[4 * k + 3]};
     if (fast) {
-        q[k] = div_nr(a[k], b[k]);
+        q[k] = fast == 2 ? div_nr_nf(a[k], b[k]) : div_nr(a[k], b[k]);
         r[k] = sqrt_nr(a[k]);
         s[k] = signal_speed<true>(u);
     } else {
@@ -935,7 +936,7 @@ struct DevBuf {
 };
 
 int mh_fast_ops(int n, const double* a, const double* b, const double* U, int fast, double* q, double* r, double* s) {
-    if (n < 0 || (n > 0 && (!a || !b || !U || !q || !r || !s)) || (fast != 0 && fast != 1))
+    if (n < 0 || (n > 0 && (!a || !b || !U || !q || !r || !s)) || fast < 0 || fast > 2)
         return fail(MH_ERR_INVALID, "mh_fast_ops: bad argument");
     if (n == 0) return MH_OK;
     const size_t nb = sizeof(double) * (size_t)n;

@@ -195,7 +195,9 @@ def _random_doubles(rng, n, lo, hi):
 
 def test_unscaled_division_and_sqrt_match_ieee(hip):
     """div_nr and sqrt_nr against the compiler's division and square root on
-    the GPU, bit for bit: 2^20 operand pairs with exponents in [-300, 300]
+    the GPU, bit for bit (the division also without its fixup, as the sweeps
+    use it for operands they know to be normal): 2^20 operand pairs with
+    exponents in [-300, 300]
     (the fast range keeps every operand within 2^+-310 and exponent gaps
     below 500, so this covers it with margin) and square roots of positive
     operands in [2^-300, 2^300].  Zero, infinite and NaN operands of the
@@ -208,7 +210,9 @@ def test_unscaled_division_and_sqrt_match_ieee(hip):
     U = np.tile([1.0, 0.5, -0.25, 3.0], (n, 1))
     qf, rf, _ = _fast_ops(a, b, U, 1)
     qi, ri, _ = _fast_ops(a, b, U, 0)
+    qn, _, _ = _fast_ops(a, b, U, 2)
     assert same_bits(qf, qi)
+    assert same_bits(qn, qi)   # the fixup is the identity for normal operands
     pos = a > 0
     assert same_bits(rf[pos], ri[pos])
     special = np.array([0.0, -0.0, np.inf, -np.inf, np.nan, 1.0, -3.5])
