@@ -183,6 +183,15 @@ def test_cyl_blast_256_device_loop(hip, oracle, solver):
     _assert_runs_match(o, g, solver)
 
 
+def test_cyl_blast_1024_full_run_device_loop(hip, oracle):
+    """The cylindrical blast at 1024^2 (scale 4) to t_end = 0.25, 339 HLLC
+    steps in the device-resident loop: dt sequence and final state bit for
+    bit equal to the oracle's."""
+    o, g = _run_both(hip, oracle, "cyl_blast", 1, scale=4, mode="device")
+    assert o["steps"] == 339
+    _assert_runs_match(o, g, 1)
+
+
 @pytest.mark.parametrize("solver", SOLVERS)
 def test_shock_bubble_nd_inflow(hip, oracle, solver):
     """Nondimensional shock-bubble with left inflow at 256x128 (scale 1/4 of
