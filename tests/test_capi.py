@@ -63,6 +63,9 @@ def test_invalid_arguments_return_status_not_crash(hip):
                       (0, 0, 1, 0, 1, 1, -1)):
         assert lib.mh_set_launch_shape(C.byref(_lib.LaunchShape(*bad_shape))) == _lib.MH_ERR_INVALID
     assert lib.mh_set_launch_shape(None) == _lib.MH_ERR_INVALID
+    x = (C.c_double * 4)()
+    assert lib.mh_fast_ops(1, x, x, x, 3, x, x, x) == _lib.MH_ERR_INVALID
+    assert lib.mh_fast_ops(1, None, x, x, 1, x, x, x) == _lib.MH_ERR_INVALID
     assert lib.mh_version().startswith(b"mh_euler")
 
 
