@@ -213,10 +213,10 @@ __device__ __forceinline__ Cons hllc(const Cons& UL, const Prim& VL, const Cons&
 // ---------------------------------------------------------------------------
 // FORCE-type flux with local alpha (head/riemann.hpp:284-325)
 // ---------------------------------------------------------------------------
-template <int D>
+template <int D, bool F = false>
 __device__ __forceinline__ Cons force(const Cons& UL, const Prim& VL, const Cons& UR, const Prim& VR) {
-    if (!phys_riemann(VL) || !phys_riemann(VR)) return hll<D>(UL, VL, UR, VR);
-    const double aL = sound(VL), aR = sound(VR);
+    if (!phys_riemann(VL) || !phys_riemann(VR)) return hll<D, F>(UL, VL, UR, VR);
+    const double aL = sound<F>(VL), aR = sound<F>(VR);
     const double unL = un_of<D>(VL), unR = un_of<D>(VR);
     const double alpha = lmax(fabs(unL) + aL, fabs(unR) + aR);
     const Cons FL = pflux<D>(UL, VL);
@@ -225,10 +225,10 @@ __device__ __forceinline__ Cons force(const Cons& UL, const Prim& VL, const Cons
     const double ha = 0.5 * alpha;
     const Cons Flf = {0.5 * (FL.r + FR.r) - ha * (UR.r - UL.r), 0.5 * (FL.mu + FR.mu) - ha * (UR.mu - UL.mu),
                       0.5 * (FL.mv + FR.mv) - ha * (UR.mv - UL.mv), 0.5 * (FL.e + FR.e) - ha * (UR.e - UL.e)};
-    const double hi = 0.5 * (1.0 / alpha);
+    const double hi = 0.5 * dvn<F>(1.0, alpha);   // alpha > 1e-14 and finite here
     const Cons Uri = {0.5 * (UL.r + UR.r) - hi * (FR.r - FL.r), 0.5 * (UL.mu + UR.mu) - hi * (FR.mu - FL.mu),
                       0.5 * (UL.mv + UR.mv) - hi * (FR.mv - FL.mv), 0.5 * (UL.e + UR.e) - hi * (FR.e - FL.e)};
-    const Prim Vri = c2p(Uri);
+    const Prim Vri = c2p<F>(Uri);
     if (!phys_riemann(Vri)) return Flf;
     const Cons Fri = pflux<D>(Uri, Vri);
     return avg(Flf, Fri);
@@ -373,13 +373,13 @@ __device__ __forceinline__ Cons exact(const Cons& UL, const Prim& VL, const Cons
     return pflux<D>(Us, Vs);
 }
 
-// F selects the unscaled division / square root for HLL and HLLC (FORCE and
-// the exact solver keep the compiler's expansions).
+// F selects the unscaled division / square root for HLL, HLLC and FORCE
+// (the exact solver keeps the compiler's expansions and pow).
 template <int S, int D, bool F = false>
 __device__ __forceinline__ Cons riemann(const Cons& UL, const Prim& VL, const Cons& UR, const Prim& VR, int& iters) {
     if (S == kHLL) return hll<D, F>(UL, VL, UR, VR);
     if (S == kHLLC) return hllc<D, F>(UL, VL, UR, VR);
-    if (S == kForce) return force<D>(UL, VL, UR, VR);
+    if (S == kForce) return force<D, F>(UL, VL, UR, VR);
     return exact<D>(UL, VL, UR, VR, iters);
 }
 
@@ -433,7 +433,7 @@ __device__ __forceinline__ void reconstruct(const Prim& Wm, const Prim& Wc, cons
 
 // Tiles whose states all lie in the fast range, swept at a dt/d in
 // [2^-20, 2^20] (dt_in_fast_range), may use div_nr / sqrt_nr in the
-// reconstruction, HLL/HLLC and the signal speed: bit-identical to the
+// reconstruction, HLL/HLLC/FORCE and the signal speed: bit-identical to the
 // compiler's division and square root there.  The fast range: rho, p and E
 // in [2^-20, 2^20).  Then E >= (mu^2 + mv^2) / (2 rho) (p > 0) bounds
 // |rho*u|, |rho*v| and |u|, |v| by 2^20.5, the reconstructed states stay
