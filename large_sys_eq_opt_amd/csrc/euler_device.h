@@ -128,11 +128,18 @@ template <int D> __device__ __forceinline__ Cons pflux(const Cons& U, const Prim
 template <int D> __device__ __forceinline__ double un_of(const Prim& V) { return D == kX ? V.u : V.v; }
 template <int D> __device__ __forceinline__ double ut_of(const Prim& V) { return D == kX ? V.v : V.u; }
 
+// Physicality checks.  In a fast tile (F = true) every state that reaches
+// one is finite wherever its density passes (the bounds at range_key), so
+// the sign tests alone give the same answer.
+template <bool F = false>
 __device__ __forceinline__ bool phys_riemann(const Prim& V) {  // head/riemann.hpp:74-81
+    if constexpr (F) return V.r > 0.0 && V.p > 0.0;
     return finite(V.r) && finite(V.u) && finite(V.v) && finite(V.p) && V.r > 0.0 && V.p > 0.0;
 }
 
+template <bool F = false>
 __device__ __forceinline__ bool phys_recon(const Prim& V) {    // src/cpu/solver_cpu.cpp:39-43
+    if constexpr (F) return (V.r > kFloor) && (V.p > kFloor);
     return finite(V.r) && finite(V.u) && finite(V.v) && finite(V.p) && (V.r > kFloor) && (V.p > kFloor);
 }
 
@@ -147,7 +154,7 @@ template <int D, bool F = false>
 __device__ __forceinline__ Cons hll(const Cons& UL, const Prim& VL, const Cons& UR, const Prim& VR) {
     const Cons FL = pflux<D>(UL, VL);
     const Cons FR = pflux<D>(UR, VR);
-    if (!phys_riemann(VL) || !phys_riemann(VR)) return avg(FL, FR);
+    if (!phys_riemann<F>(VL) || !phys_riemann<F>(VR)) return avg(FL, FR);
     const double aL = sound<F>(VL), aR = sound<F>(VR);
     const double unL = un_of<D>(VL), unR = un_of<D>(VR);
     const double SL = lmin(unL - aL, unR - aR);
@@ -183,7 +190,7 @@ __device__ __forceinline__ Cons hllc_star(const Cons& U, const Prim& V, double S
 
 template <int D, bool F = false>
 __device__ __forceinline__ Cons hllc(const Cons& UL, const Prim& VL, const Cons& UR, const Prim& VR) {
-    if (!phys_riemann(VL) || !phys_riemann(VR)) return hll<D, F>(UL, VL, UR, VR);
+    if (!phys_riemann<F>(VL) || !phys_riemann<F>(VR)) return hll<D, F>(UL, VL, UR, VR);
     const double aL = sound<F>(VL), aR = sound<F>(VR);
     const double unL = un_of<D>(VL), unR = un_of<D>(VR);
     const double pL = VL.p, pR = VR.p, rhoL = VL.r, rhoR = VR.r;
@@ -215,7 +222,7 @@ __device__ __forceinline__ Cons hllc(const Cons& UL, const Prim& VL, const Cons&
 // ---------------------------------------------------------------------------
 template <int D, bool F = false>
 __device__ __forceinline__ Cons force(const Cons& UL, const Prim& VL, const Cons& UR, const Prim& VR) {
-    if (!phys_riemann(VL) || !phys_riemann(VR)) return hll<D, F>(UL, VL, UR, VR);
+    if (!phys_riemann<F>(VL) || !phys_riemann<F>(VR)) return hll<D, F>(UL, VL, UR, VR);
     const double aL = sound<F>(VL), aR = sound<F>(VR);
     const double unL = un_of<D>(VL), unR = un_of<D>(VR);
     const double alpha = lmax(fabs(unL) + aL, fabs(unR) + aR);
@@ -229,7 +236,7 @@ __device__ __forceinline__ Cons force(const Cons& UL, const Prim& VL, const Cons
     const Cons Uri = {0.5 * (UL.r + UR.r) - hi * (FR.r - FL.r), 0.5 * (UL.mu + UR.mu) - hi * (FR.mu - FL.mu),
                       0.5 * (UL.mv + UR.mv) - hi * (FR.mv - FL.mv), 0.5 * (UL.e + UR.e) - hi * (FR.e - FL.e)};
     const Prim Vri = c2p<F>(Uri);
-    if (!phys_riemann(Vri)) return Flf;
+    if (!phys_riemann<F>(Vri)) return Flf;
     const Cons Fri = pflux<D>(Uri, Vri);
     return avg(Flf, Fri);
 }
@@ -411,7 +418,7 @@ __device__ __forceinline__ void reconstruct(const Prim& Wm, const Prim& Wc, cons
     Prim WR = {Wc.r + hs.r, Wc.u + hs.u, Wc.v + hs.v, Wc.p + hs.p};
     // The fallbacks (src/cpu/solver_cpu.cpp:86-87, :97-98) almost never fire:
     // when every lane of the wavefront passes a check the selects are skipped.
-    const bool okL = phys_recon(WL), okR = phys_recon(WR);
+    const bool okL = phys_recon<F>(WL), okR = phys_recon<F>(WR);
     if (!__all(okL && okR)) {
         if (!okL) WL = Wc;
         if (!okR) WR = Wc;
@@ -424,7 +431,7 @@ __device__ __forceinline__ void reconstruct(const Prim& Wm, const Prim& Wc, cons
     R.U = {UR.r - h.r, UR.mu - h.mu, UR.mv - h.mv, UR.e - h.e};
     L.V = c2p<F>(L.U);
     R.V = c2p<F>(R.U);
-    const bool eL = phys_recon(L.V), eR = phys_recon(R.V);
+    const bool eL = phys_recon<F>(L.V), eR = phys_recon<F>(R.V);
     if (!__all(eL && eR)) {
         if (!eL) { L.U = UL; L.V = VL; }
         if (!eR) { R.U = UR; R.V = VR; }
@@ -447,6 +454,15 @@ __device__ __forceinline__ void reconstruct(const Prim& Wm, const Prim& Wc, cons
 // 2^-310 -- none of the cases in which v_div_scale or the square root's
 // scaling changes anything (exponent gap >= 768, denormals, numerator below
 // 2^-970, argument below 2^-767).
+// The same bounds make the finiteness tests of the physicality checks
+// redundant there (phys_recon / phys_riemann with F = true): reconstructed
+// states are finite; an evolved edge state is finite below 2^110 and, when
+// its density passes (> 1e-12), so are its u, v (below 2^150) and p; a
+// Riemann input is such a state or a reconstructed state converted twice
+// (density > 1e-12 passed through unchanged); FORCE's intermediate density
+// is a difference of a value above 2^-40 and a product, so it is zero or
+// at least 2^-93 in magnitude (Sterbenz), and a positive one gives finite
+// u, v, p.  A density that fails the sign test fails either form.
 //
 // The test works on the high words (sign, exponent, top of the mantissa):
 // for x >= 0 the order of the high words is that of the values, so x lies
