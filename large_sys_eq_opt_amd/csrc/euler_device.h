@@ -180,7 +180,7 @@ __device__ __forceinline__ Cons hllc_star(const Cons& U, const Prim& V, double S
     const double denom = S - Sstar;
     if (fabs(denom) < 1.0e-14) return U;
     const double rho_star = dv<F>(rho * (S - un), denom);
-    if (!finite(rho_star) || rho_star <= 0.0) return U;
+    if ((!F && !finite(rho_star)) || rho_star <= 0.0) return U;   // finite in a fast tile
     const double E_per_rho = dvn<F>(U.e, rho);
     const double E_star = rho_star * (E_per_rho + (Sstar - un) * (Sstar + dvn<F>(p, rho * (S - un))));
     if (!finite(E_star)) return U;
@@ -204,7 +204,7 @@ __device__ __forceinline__ Cons hllc(const Cons& UL, const Prim& VL, const Cons&
     const double denominator = rhoL * (SL - unL) - rhoR * (SR - unR);
     if (fabs(denominator) < 1.0e-14) return hll<D, F>(UL, VL, UR, VR);
     const double Sstar = dv<F>(numerator, denominator);
-    if (!finite(Sstar)) return hll<D, F>(UL, VL, UR, VR);
+    if (!F && !finite(Sstar)) return hll<D, F>(UL, VL, UR, VR);   // finite in a fast tile
     // One star state is formed, on the side the sign of S* picks
     // (neighbouring faces mostly agree, so the branch rarely diverges).
     if (Sstar >= 0.0) {
@@ -418,10 +418,14 @@ __device__ __forceinline__ void reconstruct(const Prim& Wm, const Prim& Wc, cons
     Prim WR = {Wc.r + hs.r, Wc.u + hs.u, Wc.v + hs.v, Wc.p + hs.p};
     // The fallbacks (src/cpu/solver_cpu.cpp:86-87, :97-98) almost never fire:
     // when every lane of the wavefront passes a check the selects are skipped.
-    const bool okL = phys_recon<F>(WL), okR = phys_recon<F>(WR);
-    if (!__all(okL && okR)) {
-        if (!okL) WL = Wc;
-        if (!okR) WR = Wc;
+    // In a fast tile the first check always passes (the bounds at range_key),
+    // so that pass leaves it out.
+    if constexpr (!F) {
+        const bool okL = phys_recon(WL), okR = phys_recon(WR);
+        if (!__all(okL && okR)) {
+            if (!okL) WL = Wc;
+            if (!okR) WR = Wc;
+        }
     }
     const Cons UL = p2c<F>(WL), UR = p2c<F>(WR);
     const Prim VL = c2p<F>(UL), VR = c2p<F>(UR);
@@ -463,6 +467,13 @@ __device__ __forceinline__ void reconstruct(const Prim& Wm, const Prim& Wc, cons
 // is a difference of a value above 2^-40 and a product, so it is zero or
 // at least 2^-93 in magnitude (Sterbenz), and a positive one gives finite
 // u, v, p.  A density that fails the sign test fails either form.
+// The reconstruction check of WL and WR always passes there: rho and p of
+// all three cells are positive and in range, and the limited slope is at
+// most the centre's distance to a neighbour on its side, so each edge value
+// is at least half the centre value (2^-21 > 1e-12).  In HLLC the Riemann
+// inputs (rho < 2^110, |u| < 2^150, p < 2^110, a < 2^75) give |S*| < 2^461
+// and |rho*| < 2^309 (both denominators are at least 1e-14 in magnitude), so
+// those finiteness tests go too; E* keeps its test.
 //
 // The test works on the high words (sign, exponent, top of the mantissa):
 // for x >= 0 the order of the high words is that of the values, so x lies
