@@ -408,11 +408,17 @@ __device__ __forceinline__ double minmod(double a, double b) {
 
 struct Edge { Cons U; Prim V; };
 
+__device__ __forceinline__ Prim diff(const Prim& a, const Prim& b) { return {a.r - b.r, a.u - b.u, a.v - b.v, a.p - b.p}; }
+
+// MUSCL-Hancock edge states of cell c (src/cpu/solver_cpu.cpp:58-99) from
+// the backward difference dm = Wc - Wm; returns the forward difference
+// Wp - Wc.  A sweep that marches cell by cell passes it on as the next
+// cell's dm: the same subtraction of the same operands.
 template <int D, bool F = false>
-__device__ __forceinline__ void reconstruct(const Prim& Wm, const Prim& Wc, const Prim& Wp, double half, Edge& L,
-                                            Edge& R) {
-    const Prim s = {minmod(Wc.r - Wm.r, Wp.r - Wc.r), minmod(Wc.u - Wm.u, Wp.u - Wc.u),
-                    minmod(Wc.v - Wm.v, Wp.v - Wc.v), minmod(Wc.p - Wm.p, Wp.p - Wc.p)};
+__device__ __forceinline__ Prim reconstruct_d(const Prim& dm, const Prim& Wc, const Prim& Wp, double half, Edge& L,
+                                              Edge& R) {
+    const Prim dp = diff(Wp, Wc);
+    const Prim s = {minmod(dm.r, dp.r), minmod(dm.u, dp.u), minmod(dm.v, dp.v), minmod(dm.p, dp.p)};
     const Prim hs = {0.5 * s.r, 0.5 * s.u, 0.5 * s.v, 0.5 * s.p};
     Prim WL = {Wc.r - hs.r, Wc.u - hs.u, Wc.v - hs.v, Wc.p - hs.p};
     Prim WR = {Wc.r + hs.r, Wc.u + hs.u, Wc.v + hs.v, Wc.p + hs.p};
@@ -440,6 +446,13 @@ __device__ __forceinline__ void reconstruct(const Prim& Wm, const Prim& Wc, cons
         if (!eL) { L.U = UL; L.V = VL; }
         if (!eR) { R.U = UR; R.V = VR; }
     }
+    return dp;
+}
+
+template <int D, bool F = false>
+__device__ __forceinline__ void reconstruct(const Prim& Wm, const Prim& Wc, const Prim& Wp, double half, Edge& L,
+                                            Edge& R) {
+    reconstruct_d<D, F>(diff(Wc, Wm), Wc, Wp, half, L, R);
 }
 
 // Tiles whose states all lie in the fast range, swept at a dt/d in

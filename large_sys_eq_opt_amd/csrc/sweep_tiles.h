@@ -139,14 +139,17 @@ __device__ __forceinline__ bool y_tile_pass(const View& in, const View& out, int
             __hip_atomic_fetch_max(key, range_key(U.r, U.e, W.p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
     };
     m = 0.0;
-    // rows j0-2 and j0-1: convert only
+    // rows j0-2 and j0-1: convert only.  The window carries W of row p-1
+    // and the difference dm = W(p-1) - W(p-2), which is all the limiter
+    // takes of row p-2.
     Cons U0 = ld(in, ic, j0 - 2);
-    Prim W2, W1, W0;
-    conv(U0, W2);
+    Prim dm, W1, W0;
+    conv(U0, W0);
     put(0, U0);
     U0 = ld(in, ic, j0 - 1);
     conv(U0, W1);
     put(1, U0);
+    dm = diff(W1, W0);
     Cons Un = ld(in, ic, j0);
     Edge L, R, Rp;
     {   // row p = j0: reconstruct j0-1
@@ -154,8 +157,8 @@ __device__ __forceinline__ bool y_tile_pass(const View& in, const View& out, int
         Un = ld(in, ic, j0 + 1);
         conv(U0, W0);
         put(2, U0);
-        reconstruct<kY, F>(W2, W1, W0, half, L, Rp);
-        W2 = W1; W1 = W0;
+        dm = reconstruct_d<kY, F>(dm, W1, W0, half, L, Rp);
+        W1 = W0;
     }
     Cons Fp;
     {   // row p = j0+1: reconstruct j0, face j0-1/2
@@ -163,22 +166,23 @@ __device__ __forceinline__ bool y_tile_pass(const View& in, const View& out, int
         Un = ld(in, ic, j0 + 2);
         conv(U0, W0);
         put(0, U0);   // row j0-2 is no longer needed
-        reconstruct<kY, F>(W2, W1, W0, half, L, R);
+        dm = reconstruct_d<kY, F>(dm, W1, W0, half, L, R);
         Fp = riemann<S, kY, F>(Rp.U, Rp.V, L.U, L.V, it);
         Rp = R;
-        W2 = W1; W1 = W0;
+        W1 = W0;
     }
     // rows p = j0+2 .. j1+1; the ring slot of row p is (p - j0 + 2) % 3
     int slot_new = 1, slot_old = 2;   // slots of rows p and p-2
     for (int p = j0 + 2; p <= j1 + 1; ++p) {
-        U0 = Un;
+        // Row p goes into the ring slot of row p-3 (updated already) before
+        // the next row's load, so that load may reuse Un's registers.
+        conv(Un, W0);
+        put(slot_new, Un);
         Un = ld(in, ic, min(p + 1, j1 + 1));
-        conv(U0, W0);
-        reconstruct<kY, F>(W2, W1, W0, half, L, R);
+        dm = reconstruct_d<kY, F>(dm, W1, W0, half, L, R);
         const Cons Fc = riemann<S, kY, F>(Rp.U, Rp.V, L.U, L.V, it);
         const int r = p - 2;
         const Cons Unew = update(get(slot_old), dtd, Fp, Fc);
-        put(slot_new, U0);
         st(out, active ? i : kSinkCol, r, Unew);
         if (active) {
             // From rows in range, the new state's rho and p are zero or within
@@ -203,7 +207,7 @@ __device__ __forceinline__ bool y_tile_pass(const View& in, const View& out, int
         }
         Fp = Fc;
         Rp = R;
-        W2 = W1; W1 = W0;
+        W1 = W0;
         slot_old = slot_old == 2 ? 0 : slot_old + 1;
         slot_new = slot_new == 2 ? 0 : slot_new + 1;
     }
