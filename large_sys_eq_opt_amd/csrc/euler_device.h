@@ -406,6 +406,17 @@ __device__ __forceinline__ double minmod(double a, double b) {
     return (fabs(b) < fabs(a)) ? b : a;
 }
 
+// minmod as the median of a, b and 0 (four min/max instead of a product,
+// two compares and four selects).  For finite a, b whose product does not
+// underflow it returns minmod's bits, except that a zero slope may come out
+// as -0, which changes nothing where the slope is then added to or
+// subtracted from a nonzero value.  So it serves rho and p of fast tiles:
+// their differences are zero or at least 2^-72 in magnitude (in-range
+// values) and their centre values are positive.
+__device__ __forceinline__ double minmod_med(double a, double b) {
+    return __builtin_fmax(__builtin_fmin(a, b), __builtin_fmin(__builtin_fmax(a, b), 0.0));
+}
+
 struct Edge { Cons U; Prim V; };
 
 __device__ __forceinline__ Prim diff(const Prim& a, const Prim& b) { return {a.r - b.r, a.u - b.u, a.v - b.v, a.p - b.p}; }
@@ -418,7 +429,8 @@ template <int D, bool F = false>
 __device__ __forceinline__ Prim reconstruct_d(const Prim& dm, const Prim& Wc, const Prim& Wp, double half, Edge& L,
                                               Edge& R) {
     const Prim dp = diff(Wp, Wc);
-    const Prim s = {minmod(dm.r, dp.r), minmod(dm.u, dp.u), minmod(dm.v, dp.v), minmod(dm.p, dp.p)};
+    const Prim s = {F ? minmod_med(dm.r, dp.r) : minmod(dm.r, dp.r), minmod(dm.u, dp.u), minmod(dm.v, dp.v),
+                    F ? minmod_med(dm.p, dp.p) : minmod(dm.p, dp.p)};
     const Prim hs = {0.5 * s.r, 0.5 * s.u, 0.5 * s.v, 0.5 * s.p};
     Prim WL = {Wc.r - hs.r, Wc.u - hs.u, Wc.v - hs.v, Wc.p - hs.p};
     Prim WR = {Wc.r + hs.r, Wc.u + hs.u, Wc.v + hs.v, Wc.p + hs.p};
