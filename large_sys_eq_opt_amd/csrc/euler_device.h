@@ -77,7 +77,29 @@ __device__ __forceinline__ double sqrt_nr(double x) {
     d = __builtin_fma(-g, g, x);
     return __builtin_fma(d, h, g);
 }
+// dv: a quotient whose numerator may be zero or arbitrarily small -- the
+// four HLL numerators and HLLC's S* numerator, which are differences of
+// momentum and energy fluxes: a velocity component in a quiescent region or
+// on a symmetry line can be tiny (a momentum of 1e-307 is a normal double).
+// v_div_scale rescales a nonzero numerator below 2^-969 (and a denormal
+// quotient), so there the unscaled sequence can round differently.  A
+// nonzero numerator below 2^-700 takes the IEEE division instead (a lane
+// branch that is skipped when no lane needs it); zero keeps the unscaled
+// sequence, whose v_div_fixup gives the quotient its IEEE sign.  From
+// 2^-700 on, with the denominators these quotients have (at least 1e-14,
+// at most 2^263 in magnitude), the quotient is normal and no scaling
+// applies.
 template <bool F> __device__ __forceinline__ double dv(double a, double b) {
+    if constexpr (F) {
+        if (fabs(a) < 0x1p-700 && a != 0.0) return a / b;
+        return div_nr(a, b);
+    }
+    else return a / b;
+}
+// dvf: a quotient whose numerator is nonzero and bounded away from zero on
+// every path that keeps it (HLLC's rho*: |rho (S - u)| >= rho a > 2^-115),
+// keeping v_div_fixup.
+template <bool F> __device__ __forceinline__ double dvf(double a, double b) {
     if constexpr (F) return div_nr(a, b);
     else return a / b;
 }
@@ -179,7 +201,7 @@ __device__ __forceinline__ Cons hllc_star(const Cons& U, const Prim& V, double S
     const double un = un_of<D>(V), ut = ut_of<D>(V);
     const double denom = S - Sstar;
     if (fabs(denom) < 1.0e-14) return U;
-    const double rho_star = dv<F>(rho * (S - un), denom);
+    const double rho_star = dvf<F>(rho * (S - un), denom);
     if ((!F && !finite(rho_star)) || rho_star <= 0.0) return U;   // finite in a fast tile
     const double E_per_rho = dvn<F>(U.e, rho);
     const double E_star = rho_star * (E_per_rho + (Sstar - un) * (Sstar + dvn<F>(p, rho * (S - un))));
@@ -477,12 +499,16 @@ __device__ __forceinline__ void reconstruct(const Prim& Wm, const Prim& Wc, cons
 // states below 2^110 (nonzero densities above 2^-73: a difference of
 // in-range values is a multiple of their last place), and an evolved state
 // that passes the physicality check (rho, p > 1e-12) has |u| <= 2^150.
-// Every division then has a nonzero numerator above 2^-300 (or zero,
-// settled by v_div_fixup), numerator/denominator exponents less than 500
-// apart and a normal quotient, and every square root an argument above
-// 2^-310 -- none of the cases in which v_div_scale or the square root's
-// scaling changes anything (exponent gap >= 768, denormals, numerator below
-// 2^-970, argument below 2^-767).
+// Every division whose numerator is a density, pressure, energy or 1 then
+// has a nonzero numerator above 2^-300, numerator/denominator exponents
+// less than 500 apart and a normal quotient, and every square root an
+// argument above 2^-310 -- none of the cases in which v_div_scale or the
+// square root's scaling changes anything (exponent gap >= 768, denormals,
+// numerator below 2^-970, argument below 2^-767).  The range key does not
+// bound momenta from below: a velocity can be arbitrarily small, and so can
+// the HLL numerators and HLLC's S* numerator, which are differences of
+// momentum and energy fluxes.  Those quotients go through dv, which hands a
+// nonzero numerator below 2^-700 to the IEEE division.
 // The same bounds make the finiteness tests of the physicality checks
 // redundant there (phys_recon / phys_riemann with F = true): reconstructed
 // states are finite; an evolved edge state is finite below 2^110 and, when

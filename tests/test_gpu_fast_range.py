@@ -285,3 +285,94 @@ def test_fast_signal_speed_matches_ieee(hip):
     _, _, si = _fast_ops(ones, ones, U, 0)
     assert (si[:16] == 4.0).all() and (si[16:32] == 3.0).all()
     assert same_bits(sf, si)
+
+
+def tiny_momentum_field(oracle, nx, ny, seed):
+    """Density constant on 8x8-cell patches (so the density difference across
+    most faces is exactly zero) and uniform pressure, with velocities whose
+    magnitudes are drawn log-uniformly from 1e-330 to 1e-290 (below 5e-324
+    they round to zero; momenta down into the denormals), random signs.
+    Every state lies in the fast range, yet the HLL flux numerators and
+    HLLC's S* numerator -- differences of momentum and energy fluxes -- are
+    nonzero and far below 2^-969 on many faces, where the hardware
+    division's range scaling takes effect."""
+    rng = np.random.default_rng(seed)
+    shape = (ny + 4, nx + 4)
+    y, x = np.mgrid[0:ny + 4, 0:nx + 4]
+    rho = 0.5 + 0.5 * ((x // 8 + 3 * (y // 8)) % 5) / 4.0
+    mag = lambda: 10.0 ** rng.uniform(-330, -290, shape) * rng.choice([-1.0, 1.0], shape)
+    V = np.stack([rho, mag(), mag(), np.ones(shape)], axis=-1)
+    return oracle.prim_to_cons(V.reshape(-1, 4)).reshape(ny + 4, nx + 4, 4)
+
+
+def hll_density_numerators(oracle, U, direction, d, dt):
+    """The density component of the HLL numerator SR FL - SL FR + SL SR (UR - UL)
+    at every face between interior neighbours along `direction` (numpy
+    arithmetic on the oracle's edge states: its magnitude is what matters)."""
+    sl = (slice(2, -2), slice(1, -3)) if direction == 0 else (slice(1, -3), slice(2, -2))
+    sp = (slice(2, -2), slice(3, -1)) if direction == 0 else (slice(3, -1), slice(2, -2))
+    sc = (slice(2, -2), slice(2, -2))
+    L, R = oracle.reconstruct(U[sl].reshape(-1, 4), U[sc].reshape(-1, 4), U[sp].reshape(-1, 4), dt / d, direction)
+    n = U[sc].shape[:2]
+    L, R = L.reshape(n + (4,)), R.reshape(n + (4,))
+    if direction == 0:
+        UL, UR = R[:, :-1].reshape(-1, 4), L[:, 1:].reshape(-1, 4)
+    else:
+        UL, UR = R[:-1, :].reshape(-1, 4), L[1:, :].reshape(-1, 4)
+    m = 1 + direction
+
+    def speeds(S):
+        un = S[:, m] / S[:, 0]
+        p = 0.4 * (S[:, 3] - 0.5 * (S[:, 1] ** 2 + S[:, 2] ** 2) / S[:, 0])
+        return un, np.sqrt(1.4 * p / S[:, 0])
+
+    (uL, aL), (uR, aR) = speeds(UL), speeds(UR)
+    SL, SR = np.minimum(uL - aL, uR - aR), np.maximum(uL + aL, uR + aR)
+    return SR * UL[:, m] - SL * UR[:, m] + SL * SR * (UR[:, 0] - UL[:, 0])
+
+
+@pytest.mark.parametrize("solver", BITWISE_SOLVERS)
+@pytest.mark.parametrize("direction", [0, 1])
+def test_sweep_tiny_momenta_bitwise(hip, oracle, solver, direction):
+    """A sweep of the tiny-momentum field: fast tiles throughout, and their
+    quotients with tiny nonzero numerators must still equal the oracle's bit
+    for bit (the HLL numerators themselves, HLLC's S* and its HLL fallback,
+    FORCE's HLL fallback)."""
+    from large_sys_eq_opt_amd import Grid2DGPU, sweep_gpu
+
+    nx, ny = 130, 97
+    U = tiny_momentum_field(oracle, nx, ny, 21 + direction)
+    assert in_fast_range_fraction(U) == 1.0
+    dx, dy = 1.0 / nx, 1.0 / ny
+    dt = 0.3 * min(dx, dy)
+    num = np.abs(hll_density_numerators(oracle, U, direction, dx if direction == 0 else dy, dt))
+    assert ((num > 0) & (num < 2.0 ** -969)).mean() > 0.5
+    ref = oracle.sweep(U, dx, dy, dt, direction, solver)
+    a = Grid2DGPU(nx, ny, 2, 0.0, 1.0, 0.0, 1.0)
+    b = Grid2DGPU(nx, ny, 2, 0.0, 1.0, 0.0, 1.0)
+    a.upload_from_aos(U)
+    b.upload_from_aos(U)
+    sweep_gpu(a, b, dt, direction, solver)
+    assert same_bits(interior(b.download_to_aos()), interior(ref))
+
+
+@pytest.mark.parametrize("solver", BITWISE_SOLVERS)
+def test_device_loop_tiny_momenta_bitwise(hip, oracle, solver):
+    """Twelve device-loop steps from the tiny-momentum field: dt sequence
+    and state bit for bit."""
+    from large_sys_eq_opt_amd import Grid2DGPU, Stepper, cases
+
+    nx, ny, steps = 300, 260, 12
+    cfg = cases.CaseConfig(nx, ny, 2, 0.0, 1.0, 0.0, 1.0, 0.8, 0.0)
+    U0 = tiny_momentum_field(oracle, nx, ny, 5)
+    o = oracle.run(U0, cfg.dx, cfg.dy, solver, cfg.cfl, max_steps=steps)
+    g = Grid2DGPU.from_config(cfg)
+    g.upload_from_aos(U0)
+    st = Stepper(g, solver, cfg.cfl, 0.0, None, log_capacity=steps)
+    st.advance(steps)
+    st.finish()
+    _, n, dts = st.clock()
+    out = g.download_to_aos()
+    assert n == steps
+    assert same_bits(dts, o["dt"])
+    assert same_bits(interior(out), interior(o["U"]))

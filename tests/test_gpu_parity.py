@@ -201,6 +201,40 @@ def test_shock_bubble_nd_inflow(hip, oracle, solver):
     _assert_runs_match(o, g, solver)
 
 
+@pytest.mark.parametrize("solver", SOLVERS)
+def test_shock_bubble_1024x512_full_run(hip, oracle, solver):
+    """The Mach 1.22 shock-bubble case at its full size (1024x512 over
+    [0,2]x[0,1], left Dirichlet inflow, transmissive elsewhere) run to
+    t_end = 0.4 with 5 snapshots in the reference-compatible host loop
+    (about 940-950 steps): step count, dt sequence, snapshot times, every
+    snapshot and the final state against the oracle -- bit for bit for HLL,
+    HLLC and FORCE; exact to 1e-10 relative state / 1e-12 relative dt."""
+    from large_sys_eq_opt_amd import cases
+    from large_sys_eq_opt_amd.driver import run_case
+
+    case = "shock_bubble_nd"
+    cfg = cases.get_case_config(case)
+    assert (cfg.nx, cfg.ny, cfg.t_end) == (1024, 512, 0.4)
+    kind, params = cases.case_params(case)
+    U0 = oracle.init(kind, params.array(), cfg.nx, cfg.ny, cfg.bounds)
+    inflow = cases.case_boundary(case).left_inflow_state
+    o = oracle.run(U0, cfg.dx, cfg.dy, solver, cfg.cfl, t_end=cfg.t_end, inflow_left=inflow, nsnap=5,
+                   keep_snapshots=True)
+    g = run_case(case, solver, 1, num_snapshots=5, keep_snapshots=True)
+    assert 900 < o["steps"] < 1000
+    _assert_runs_match(o, g, solver)
+    assert len(g["snapshots"]) == 5
+    if BITWISE[solver]:
+        assert same_bits(g["snap_t"], o["snap_t"])
+    else:
+        assert np.max(np.abs(g["snap_t"] - o["snap_t"])) <= 1e-12
+    for k in range(5):
+        if BITWISE[solver]:
+            assert same_bits(interior(g["snapshots"][k]), interior(o["snapshots"][k])), f"snapshot {k + 1}"
+        else:
+            assert rel_linf(g["snapshots"][k], o["snapshots"][k]) <= EXACT_STATE_TOL, f"snapshot {k + 1}"
+
+
 # Launch shapes: the default, short/long/whole-line segments, both LDS
 # layouts, 1 and 4 tiles per workgroup, and the small-grid segment shrink.
 LAUNCH_SHAPES = [
