@@ -87,3 +87,33 @@ def test_batch_other_solvers(hip, oracle, solver):
         else:
             assert same_bits(interior(U), interior(o["U"]))
     b.close()
+
+
+def test_batch_ensemble_512_seed0_500_steps(hip, oracle):
+    """The 512-instance ensemble from seed 0 (sides 64..1024, 95.3 M cells)
+    advanced 500 HLLC steps, 25 steps per captured graph, under the
+    dispatched schedule the benchmark's tuner picks and the persistent one:
+    every instance takes 500 steps with status 0, and the smallest, the
+    largest and six seeded random instances equal their own oracle runs bit
+    for bit (state, last dt and t).  In 500 steps the blast waves of the
+    smaller instances cross the transmissive edges."""
+    from large_sys_eq_opt_amd.ensemble import Batch, ensemble_spec
+
+    spec = ensemble_spec(512, 0)
+    sides = np.array([s["side"] for s in spec])
+    rng = np.random.default_rng(3)
+    picks = sorted({int(np.argmin(sides)), int(np.argmax(sides)), *rng.choice(len(spec), 6, replace=False).tolist()})
+    ref = {k: oracle_instance(oracle, spec[k], 1, 500) for k in picks}
+    for schedule in [(0, 128, 2, 0, 4), (0, 256, 0, 0, 1)]:
+        b = Batch(spec, solver=1)
+        b.set_schedule(*schedule)
+        b.initialise()
+        b.capture(25)
+        b.step(500)
+        t, dt, steps, status = b.clock()
+        assert (steps == 500).all() and (status == 0).all(), schedule
+        for k in picks:
+            o = ref[k]
+            assert same_bits(interior(b.download(k)), interior(o["U"])), (schedule, k, spec[k])
+            assert dt[k] == o["dt"][-1] and t[k] == np.float64(sum_seq(o["dt"])), (schedule, k)
+        b.close()

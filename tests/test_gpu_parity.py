@@ -202,6 +202,24 @@ def test_shock_bubble_nd_inflow(hip, oracle, solver):
 
 
 @pytest.mark.parametrize("solver", SOLVERS)
+def test_cyl_blast_256_through_boundaries(hip, oracle, solver):
+    """The 256^2 cylindrical blast run to t = 1.0, four times its case's
+    t_end: the blast wave leaves through the transmissive edges and corners
+    and the reflected-free outflow keeps evolving (device-resident loop)."""
+    from large_sys_eq_opt_amd import cases
+    from large_sys_eq_opt_amd.driver import run_case
+
+    cfg = cases.get_case_config("cyl_blast")
+    kind, params = cases.case_params("cyl_blast")
+    U0 = oracle.init(kind, params.array(), cfg.nx, cfg.ny, cfg.bounds)
+    o = oracle.run(U0, cfg.dx, cfg.dy, solver, cfg.cfl, t_end=1.0)
+    g = run_case("cyl_blast", solver, 1, mode="device", t_end=1.0)
+    edge = np.abs(interior(o["U"])[[0, -1], :, 1:3]).max()
+    assert edge > 1e-2, "the blast has reached the edges"
+    _assert_runs_match(o, g, solver)
+
+
+@pytest.mark.parametrize("solver", SOLVERS)
 def test_shock_bubble_1024x512_full_run(hip, oracle, solver):
     """The Mach 1.22 shock-bubble case at its full size (1024x512 over
     [0,2]x[0,1], left Dirichlet inflow, transmissive elsewhere) run to
