@@ -156,7 +156,7 @@ typedef struct {
     int seg_x;    /* x tile: cells per row segment (0 = whole row) */
     int seg_y;    /* y tile: rows per chunk (0 = whole column) */
     int y_waves;  /* y sweep register target: 4 wavefronts/SIMD, or 1 = compiler's choice */
-    int x_layout; /* x tile LDS exchange: 0 compact (6.3 KB, selects), 1 66-slot (10.3 KB) */
+    int x_layout; /* x tile LDS exchange: 0 compact (6.3 KB, selects), 1 66-slot (10.3 KB), 2 compact with the update deferred one iteration (two LDS round trips instead of three) */
     int x_wpb;    /* wavefronts (tiles) per workgroup, x sweep: 1 or 4 */
     int y_wpb;    /* wavefronts (tiles) per workgroup, y sweep: 1 or 4 */
     int min_tiles; /* a sweep whose segments give fewer tiles than this gets

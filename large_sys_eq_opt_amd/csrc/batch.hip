@@ -148,7 +148,7 @@ __global__ __launch_bounds__(256) void k_batch_step(BatchArgs a) {
             double dt;
             if (!inst_dt(I, a.clk[t.inst], par, dt)) continue;
             const double dtd = dt / I.dx;
-            for (int j = t.j0; j < t.j1; ++j) x_tile<S, 0>(I.U, I.T, j, t.i0, t.i1, dtd, 0.5 * dtd, I.bc, 1, X[w].x, lane);
+            for (int j = t.j0; j < t.j1; ++j) x_tile<S, 2>(I.U, I.T, j, t.i0, t.i1, dtd, 0.5 * dtd, I.bc, 1, X[w].x, lane);
         }
     }
     if (a.phases == 3) grid_barrier(&a.ctrl->bar[par], gridDim.x, &a.ctrl->err);
@@ -190,7 +190,7 @@ __global__ __launch_bounds__(64) void k_batch_x(BatchArgs a, int ntiles) {
     double dt;
     if (!inst_dt(I, a.clk[t.inst], par, dt)) return;
     const double dtd = dt / I.dx;
-    for (int j = t.j0; j < t.j1; ++j) x_tile<S, 0>(I.U, I.T, j, t.i0, t.i1, dtd, 0.5 * dtd, I.bc, 1, X[w], lane);
+    for (int j = t.j0; j < t.j1; ++j) x_tile<S, 2>(I.U, I.T, j, t.i0, t.i1, dtd, 0.5 * dtd, I.bc, 1, X[w], lane);
 }
 
 // Dispatched y phase: wavefront q runs y tile q with the fast division

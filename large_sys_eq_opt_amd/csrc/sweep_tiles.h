@@ -386,7 +386,108 @@ __device__ __forceinline__ bool x_tilec_pass(const View& in, const View& out, in
     return true;
 }
 
-// Layout selector: L = 0 compact, 1 = 66-slot.
+// The x tile with the update deferred by one iteration, over the compact
+// slice: the face flux of iteration k goes to LDS (M[4..7], which the next
+// iteration's primitive exchange leaves alone) and is read, with the
+// conserved state of the cell two lanes left, in iteration k+1's primitive
+// exchange; the cell update of iteration k runs there.  Two LDS round trips
+// per iteration instead of three; the own face flux stays in registers
+// across the next iteration's conversion only.  The U slots hold the
+// previous iteration's states until that read is done, so lanes 62-63 copy
+// theirs into the carry before every lane writes its new one.
+template <int S, bool F>
+__device__ __forceinline__ bool x_tiled_pass(const View& in, const View& out, int j, int i0, int i1, double dtd,
+                                             double half, const Bc& bc, int ghosts, XLdsC& X, int lane) {
+    int it = 0;
+    const int niter = (i1 - i0 + 4 + 63) / 64;
+    const bool l1 = lane >= 1, l2 = lane >= 2;
+    unsigned key = 0;   // running range key of the states read (fast pass)
+    Cons Fprev = {0.0, 0.0, 0.0, 0.0};
+    // the update of the cell two left of lane `lane`'s cell of iteration kk
+    auto update_prev = [&](int kk, const Cons& Fm, const Cons& U2) {
+        const int r = i0 - 4 + 64 * kk + lane;
+        const bool mine = r >= i0 && r < i1;
+        const Cons Unew = update(U2, dtd, Fm, Fprev);
+        st(out, mine ? r : kSinkCol, j, Unew);
+        if (mine && ghosts) {
+            if (j == 0) {
+                const Cons G = bc.kind[MH_EDGE_BOTTOM] == MH_BC_INFLOW ? bc.s[MH_EDGE_BOTTOM] : Unew;
+                st(out, r, -1, G);
+                st(out, r, -2, G);
+            }
+            if (j == in.ny - 1) {
+                const Cons G = bc.kind[MH_EDGE_TOP] == MH_BC_INFLOW ? bc.s[MH_EDGE_TOP] : Unew;
+                st(out, r, in.ny, G);
+                st(out, r, in.ny + 1, G);
+            }
+        }
+    };
+    // reads the previous iteration's left face flux and the conserved state
+    // two lanes left (carries for lanes 0-1)
+    auto read_prev = [&](Cons& Fm, Cons& U2) {
+        double f1[4], u2[4];
+        for (int c = 0; c < 4; ++c) {
+            f1[c] = l1 ? X.M[4 + c][lane - 1] : X.cF[c];
+            u2[c] = l2 ? X.U[c][lane - 2] : X.cU[c][lane];
+        }
+        Fm = {f1[0], f1[1], f1[2], f1[3]};
+        U2 = {u2[0], u2[1], u2[2], u2[3]};
+    };
+    Cons Un = ld(in, min(i0 - 2 + lane, in.nx + 1), j);
+    for (int k = 0; k < niter; ++k) {
+        const int p = i0 - 2 + 64 * k + lane;
+        const Cons U0 = Un;
+        Un = ld(in, min(p + 64, in.nx + 1), j);
+        const Prim W0 = c2p<F>(U0);
+        if constexpr (F) key = max(key, range_key(U0.r, U0.e, W0.p));
+        X.M[0][lane] = W0.r; X.M[1][lane] = W0.u; X.M[2][lane] = W0.v; X.M[3][lane] = W0.p;
+        wave_sync();
+        double q1[4], q2[4];
+        for (int c = 0; c < 4; ++c) {
+            q1[c] = l1 ? X.M[c][lane - 1] : X.cW[c][1];
+            q2[c] = l2 ? X.M[c][lane - 2] : X.cW[c][lane];
+        }
+        Cons Fm, U2;
+        read_prev(Fm, U2);
+        wave_sync();
+        if (lane >= 62) {
+            X.cW[0][lane - 62] = W0.r; X.cW[1][lane - 62] = W0.u; X.cW[2][lane - 62] = W0.v;
+            X.cW[3][lane - 62] = W0.p;
+            for (int c = 0; c < 4; ++c) X.cU[c][lane - 62] = X.U[c][lane];
+        }
+        X.U[0][lane] = U0.r; X.U[1][lane] = U0.mu; X.U[2][lane] = U0.mv; X.U[3][lane] = U0.e;
+        if (lane == 63) { X.cF[0] = Fprev.r; X.cF[1] = Fprev.mu; X.cF[2] = Fprev.mv; X.cF[3] = Fprev.e; }
+        if (k > 0) update_prev(k - 1, Fm, U2);
+        const Prim W1 = {q1[0], q1[1], q1[2], q1[3]};
+        const Prim W2 = {q2[0], q2[1], q2[2], q2[3]};
+        Edge L, R;
+        reconstruct<kX, F>(W2, W1, W0, half, L, R);
+        X.M[0][lane] = R.U.r; X.M[1][lane] = R.U.mu; X.M[2][lane] = R.U.mv; X.M[3][lane] = R.U.e;
+        X.M[4][lane] = R.V.r; X.M[5][lane] = R.V.u; X.M[6][lane] = R.V.v; X.M[7][lane] = R.V.p;
+        wave_sync();
+        double r1[8];
+        for (int c = 0; c < 8; ++c) r1[c] = l1 ? X.M[c][lane - 1] : X.cR[c];
+        wave_sync();
+        if (lane == 63) {
+            X.cR[0] = R.U.r; X.cR[1] = R.U.mu; X.cR[2] = R.U.mv; X.cR[3] = R.U.e;
+            X.cR[4] = R.V.r; X.cR[5] = R.V.u; X.cR[6] = R.V.v; X.cR[7] = R.V.p;
+        }
+        const Cons RU = {r1[0], r1[1], r1[2], r1[3]};
+        const Prim RV = {r1[4], r1[5], r1[6], r1[7]};
+        Fprev = riemann<S, kX, F>(RU, RV, L.U, L.V, it);
+        X.M[4][lane] = Fprev.r; X.M[5][lane] = Fprev.mu; X.M[6][lane] = Fprev.mv; X.M[7][lane] = Fprev.e;
+    }
+    wave_sync();
+    Cons Fm, U2;
+    read_prev(Fm, U2);
+    update_prev(niter - 1, Fm, U2);
+    wave_sync();   // the slice is free for the next tile of this wavefront
+    if constexpr (F) return !__any(key >= kRangeKeyEnd);
+    return true;
+}
+
+// Layout selector: L = 0 compact, 1 = 66-slot, 2 = compact with the update
+// deferred by one iteration.
 template <int L> struct XSlice { using type = XLdsC; };
 template <> struct XSlice<1> { using type = XLds66; };
 
@@ -399,6 +500,9 @@ __device__ __forceinline__ void x_tile(const View& in, const View& out, int j, i
     if constexpr (L == 1) {
         if (dt_in_fast_range(dtd) && x_tile66_pass<S, true>(in, out, j, i0, i1, dtd, half, bc, ghosts, X, lane)) return;
         x_tile66_pass<S, false>(in, out, j, i0, i1, dtd, half, bc, ghosts, X, lane);
+    } else if constexpr (L == 2) {
+        if (dt_in_fast_range(dtd) && x_tiled_pass<S, true>(in, out, j, i0, i1, dtd, half, bc, ghosts, X, lane)) return;
+        x_tiled_pass<S, false>(in, out, j, i0, i1, dtd, half, bc, ghosts, X, lane);
     } else {
         if (dt_in_fast_range(dtd) && x_tilec_pass<S, true>(in, out, j, i0, i1, dtd, half, bc, ghosts, X, lane)) return;
         x_tilec_pass<S, false>(in, out, j, i0, i1, dtd, half, bc, ghosts, X, lane);

@@ -141,6 +141,7 @@ SHAPES = [
     dict(seg_x=1020, seg_y=128, y_waves=1, x_layout=0, x_wpb=1, y_wpb=4, min_tiles=0),
     dict(seg_x=60, seg_y=16, y_waves=4, x_layout=1, x_wpb=4, y_wpb=1, min_tiles=0),
     dict(seg_x=1020, seg_y=128, y_waves=1, x_layout=0, x_wpb=1, y_wpb=4, min_tiles=2048),
+    dict(seg_x=100, seg_y=128, y_waves=1, x_layout=2, x_wpb=4, y_wpb=4, min_tiles=0),
 ]
 
 

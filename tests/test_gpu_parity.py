@@ -253,7 +253,7 @@ def test_shock_bubble_1024x512_full_run(hip, oracle, solver):
             assert rel_linf(g["snapshots"][k], o["snapshots"][k]) <= EXACT_STATE_TOL, f"snapshot {k + 1}"
 
 
-# Launch shapes: the default, short/long/whole-line segments, both LDS
+# Launch shapes: the default, short/long/whole-line segments, the three LDS
 # layouts, 1 and 4 tiles per workgroup, and the small-grid segment shrink.
 LAUNCH_SHAPES = [
     dict(seg_x=1020, seg_y=128, y_waves=1, x_layout=0, x_wpb=1, y_wpb=4, min_tiles=0),
@@ -261,6 +261,9 @@ LAUNCH_SHAPES = [
     dict(seg_x=0, seg_y=0, y_waves=1, x_layout=1, x_wpb=1, y_wpb=1, min_tiles=0),
     dict(seg_x=100, seg_y=37, y_waves=4, x_layout=0, x_wpb=4, y_wpb=4, min_tiles=0),
     dict(seg_x=1020, seg_y=128, y_waves=1, x_layout=0, x_wpb=1, y_wpb=4, min_tiles=2048),
+    dict(seg_x=1020, seg_y=128, y_waves=1, x_layout=2, x_wpb=1, y_wpb=4, min_tiles=0),
+    dict(seg_x=60, seg_y=16, y_waves=4, x_layout=2, x_wpb=4, y_wpb=1, min_tiles=0),
+    dict(seg_x=100, seg_y=37, y_waves=1, x_layout=2, x_wpb=1, y_wpb=4, min_tiles=0),
 ]
 
 
