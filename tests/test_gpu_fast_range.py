@@ -377,3 +377,53 @@ def test_device_loop_tiny_momenta_bitwise(hip, oracle, solver):
     assert n == steps
     assert same_bits(dts, o["dt"])
     assert same_bits(interior(out), interior(o["U"]))
+
+
+def extreme_in_range_field(oracle, nx, ny, seed):
+    """Uncorrelated cell states spread over the whole fast range: rho and p
+    log-uniform in [2^-20, 2^20) (p below 2^18.6 so that p/(g-1) stays in
+    range), velocity components with magnitudes log-uniform from 1e-320 to
+    the bound E < 2^20 leaves (a third of them exactly zero), random signs.
+    Neighbours differ by up to 2^40 in density and pressure, so the
+    reconstruction and the Riemann solvers take their fallbacks often, with
+    every state still in the range where fast tiles keep their results."""
+    rng = np.random.default_rng(seed)
+    shape = (ny + 4, nx + 4)
+    rho = 2.0 ** rng.uniform(-20, 19.9, shape)
+    p = 2.0 ** rng.uniform(-20, 18.5, shape)
+    vmax = np.sqrt(0.9 * (2.0 ** 20 - p / 0.4) / rho)
+
+    def vel():
+        mag = np.minimum(10.0 ** rng.uniform(-320, 3.5, shape), vmax / np.sqrt(2.0))
+        v = mag * rng.choice([-1.0, 1.0], shape)
+        v[rng.random(shape) < 1 / 3] = 0.0
+        return v
+
+    V = np.stack([rho, vel(), vel(), p], axis=-1)
+    return oracle.prim_to_cons(V.reshape(-1, 4)).reshape(ny + 4, nx + 4, 4)
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+@pytest.mark.parametrize("solver", BITWISE_SOLVERS)
+@pytest.mark.parametrize("direction", [0, 1])
+def test_sweep_extreme_in_range_states_bitwise(hip, oracle, seed, solver, direction):
+    """Sweeps of uncorrelated states spread over the whole fast range, at
+    dt/d of 2^-19, 0.37 and 2^19 (the first and last near the ends of the
+    dt range the fast tiles accept): bit for bit equal to the oracle."""
+    from large_sys_eq_opt_amd import Grid2DGPU, sweep_gpu
+
+    nx, ny = 150, 140
+    U = extreme_in_range_field(oracle, nx, ny, 100 * seed + 10 * solver + direction)
+    assert in_fast_range_fraction(U) > 0.99
+    dx, dy = 1.0 / nx, 1.0 / ny
+    d = dx if direction == 0 else dy
+    for dtd in (2.0 ** -19, 0.37, 2.0 ** 19):
+        ref = oracle.sweep(U, dx, dy, dtd * d, direction, solver)
+        a = Grid2DGPU(nx, ny, 2, 0.0, 1.0, 0.0, 1.0)
+        b = Grid2DGPU(nx, ny, 2, 0.0, 1.0, 0.0, 1.0)
+        a.upload_from_aos(U)
+        b.upload_from_aos(U)
+        sweep_gpu(a, b, dtd * d, direction, solver)
+        assert same_bits(interior(b.download_to_aos()), interior(ref)), dtd
+        a.close()
+        b.close()
