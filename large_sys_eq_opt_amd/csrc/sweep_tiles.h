@@ -82,7 +82,8 @@ struct XLds66 {
 // wave_sync, so they share it); each quantity keeps the values the next
 // iteration needs from lanes 62-63 in its own carry slots, and lanes 0-1
 // select between neighbour and carry.  Fewer bytes, more selects: which
-// layout is faster depends on occupancy, so both exist.
+// layout is faster depends on occupancy, so both exist.  The deferred-update
+// x tile (x_tiled_pass, the default) uses this slice too.
 struct XLdsC {
     double M[8][64];
     double U[4][64];
