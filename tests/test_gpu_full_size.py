@@ -1,6 +1,7 @@
-"""Size-independent properties at the benchmark's full sizes (8192^2 and
-16384^2 HLLC), where the oracle would take hours: the state stays on the
-device and the checks run there (torch on the copied interior).
+"""The benchmark's full sizes (8192^2 and 16384^2).  A few steps against the
+oracle (its OpenMP sweeps take seconds at these sizes; whole runs would take
+hours), and size-independent properties over more steps, with the state on
+the device and the checks run there (torch on the copied interior):
 
   * mirror symmetry of a centred cylindrical blast about both centre lines,
     to 1e-12 of each field's largest magnitude (the scheme is mirror
@@ -186,3 +187,33 @@ def test_tensor_copies_check_dtype_and_shape(hip):
     torch.cuda.synchronize()
     assert torch.equal(back, ok)
     g.close()
+
+
+@pytest.mark.parametrize("n, solver, steps", [(8192, 1, 3), (8192, 2, 2), (16384, 1, 1)])
+def test_full_size_steps_match_oracle(hip, oracle, n, solver, steps):
+    """The benchmark's own workload at full size -- the cylindrical blast on
+    n x n cells through the device-resident loop -- for a few steps against
+    the oracle on the same initial state (its OpenMP sweeps take seconds
+    here): dt sequence and state bit for bit for HLLC, to 1e-12 (dt) and
+    1e-10 (state, relative) for the exact solver, whose device pow may
+    differ in the last bit."""
+    import numpy as np
+
+    from conftest import interior, rel_linf, same_bits
+    from large_sys_eq_opt_amd import cases
+    from large_sys_eq_opt_amd.driver import run_case
+
+    cfg = cases.get_n_case_config("cyl_blast", n // 256)
+    assert (cfg.nx, cfg.ny) == (n, n)
+    kind, params = cases.case_params("cyl_blast")
+    U0 = oracle.init(kind, params.array(), cfg.nx, cfg.ny, cfg.bounds)
+    o = oracle.run(U0, cfg.dx, cfg.dy, solver, cfg.cfl, max_steps=steps)
+    del U0
+    g = run_case("cyl_blast", solver, n // 256, mode="device", max_steps=steps)
+    assert o["steps"] == g["steps"] == steps
+    if solver == 2:
+        assert np.max(np.abs(g["dt"] - o["dt"]) / o["dt"]) <= 1e-12
+        assert rel_linf(g["U"], o["U"]) <= 1e-10
+    else:
+        assert same_bits(g["dt"], o["dt"])
+        assert same_bits(interior(g["U"]), interior(o["U"]))

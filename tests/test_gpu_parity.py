@@ -151,10 +151,11 @@ def _run_both(hip, oracle, case, solver, scale=1, mode="host", nsnap=0, max_step
     kind, params = cases.case_params(case)
     U0 = oracle.init(kind, params.array(), cfg.nx, cfg.ny, cfg.bounds)
     inflow = cases.case_boundary(case).left_inflow_state
+    keep = bool(nsnap) and mode == "host"
     o = oracle.run(U0, cfg.dx, cfg.dy, solver, cfg.cfl, t_end=cfg.t_end, inflow_left=inflow, nsnap=nsnap,
-                   max_steps=max_steps if max_steps else 1 << 30)
+                   max_steps=max_steps if max_steps else 1 << 30, keep_snapshots=keep)
     g = run_case(case, solver, scale, mode=mode, num_snapshots=nsnap, max_steps=max_steps,
-                 t_end=cfg.t_end if max_steps else None)
+                 t_end=cfg.t_end if max_steps else None, keep_snapshots=keep)
     return o, g
 
 
@@ -170,10 +171,18 @@ def _assert_runs_match(o, g, solver):
 
 @pytest.mark.parametrize("solver", SOLVERS)
 def test_cyl_blast_256_full_run(hip, oracle, solver):
-    """Cylindrical blast 256^2 to t = 0.25 with 5 snapshots (host-synchronous loop)."""
+    """Cylindrical blast 256^2 to t = 0.25 with 5 snapshots (host-synchronous loop):
+    step count, dt sequence, snapshot times, every snapshot state and the final
+    state against the oracle."""
     o, g = _run_both(hip, oracle, "cyl_blast", solver, nsnap=5)
     _assert_runs_match(o, g, solver)
     assert np.array_equal(g["snap_t"], o["snap_t"]) or not BITWISE[solver]
+    assert len(g["snapshots"]) == len(o["snapshots"]) == 5
+    for k in range(5):
+        if BITWISE[solver]:
+            assert same_bits(interior(g["snapshots"][k]), interior(o["snapshots"][k])), f"snapshot {k + 1}"
+        else:
+            assert rel_linf(g["snapshots"][k], o["snapshots"][k]) <= EXACT_STATE_TOL, f"snapshot {k + 1}"
 
 
 @pytest.mark.parametrize("solver", [1, 2])
