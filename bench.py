@@ -85,7 +85,7 @@ def batched_leg(plain_rate: float, steps: int = 500):
     spec = ensemble.ensemble_spec(512, 0)
     b = ensemble.Batch(spec, solver=1, cfl=0.8)
     b.initialise()
-    tune = ensemble.autotune(b, steps=4)
+    tune = ensemble.autotune(b)
     per_graph = 25
     b.capture(per_graph)
     b.clock()
