@@ -330,3 +330,38 @@ def test_first_order_run(hip, oracle, solver):
         o = oracle.run(U0, cfg.dx, cfg.dy, solver, 0.4, order=1, inflow_left=inflow, max_steps=40)
         g = run_case(case, solver, 1, mode="host", order=1, max_steps=40, t_end=0.0, cfl=0.4)
         _assert_runs_match(o, g, solver)
+
+
+# The reference's own runs at scale 1 and their recorded total step counts
+# (validation/cpu_mpi_fullnode_external_timing_11488.csv:2-16).
+REFERENCE_STEP_COUNTS = {("shock_bubble", 0): 6359, ("shock_bubble", 1): 6913, ("shock_bubble", 2): 6912,
+                         ("shock_bubble", 3): 6539, ("blast_wave", 0): 2306, ("blast_wave", 1): 2335,
+                         ("blast_wave", 2): 2334, ("blast_wave", 3): 2308}
+
+
+@pytest.mark.parametrize("case, solver", sorted(REFERENCE_STEP_COUNTS))
+def test_reference_step_counts_on_gpu(hip, case, solver):
+    """The reference's two cases -- the air/helium shock-bubble in SI units
+    (500x197) and the blast wave (500x500) -- run to their t_end through the
+    device-resident loop take exactly the step counts the reference recorded
+    for each solver (one rounding of any dt can change the count)."""
+    from large_sys_eq_opt_amd.driver import run_case
+
+    g = run_case(case, solver, 1, mode="device")
+    assert g["steps"] == REFERENCE_STEP_COUNTS[(case, solver)]
+
+
+@pytest.mark.parametrize("case", ["shock_bubble", "blast_wave"])
+def test_reference_cases_match_oracle(hip, oracle, case):
+    """The same HLLC runs against the oracle: dt sequence and final state bit
+    for bit (about 6900 and 2300 steps)."""
+    from large_sys_eq_opt_amd import cases
+    from large_sys_eq_opt_amd.driver import run_case
+
+    cfg = cases.get_case_config(case)
+    kind, params = cases.case_params(case)
+    U0 = oracle.init(kind, params.array(), cfg.nx, cfg.ny, cfg.bounds)
+    o = oracle.run(U0, cfg.dx, cfg.dy, 1, cfg.cfl, t_end=cfg.t_end)
+    g = run_case(case, 1, 1, mode="device")
+    assert g["steps"] == o["steps"] == REFERENCE_STEP_COUNTS[(case, 1)]
+    _assert_runs_match(o, g, 1)
