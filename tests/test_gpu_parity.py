@@ -365,3 +365,15 @@ def test_reference_cases_match_oracle(hip, oracle, case):
     g = run_case(case, 1, 1, mode="device")
     assert g["steps"] == o["steps"] == REFERENCE_STEP_COUNTS[(case, 1)]
     _assert_runs_match(o, g, 1)
+
+
+def test_reference_snapshot_times_on_gpu(hip):
+    """The reference's printed snapshot times of the shock-bubble HLL run
+    (outputs/baseline/cpu_stdout.txt:1-5, 6 significant digits), from the
+    host-synchronous loop with 5 snapshots (first step with t >= k t_end / 5)."""
+    from large_sys_eq_opt_amd.driver import run_case
+
+    g = run_case("shock_bubble", 0, 1, num_snapshots=5)
+    assert g["steps"] == REFERENCE_STEP_COUNTS[("shock_bubble", 0)]
+    rec = [0.000234916, 0.000469817, 0.000704535, 0.000939401, 0.0011741]
+    assert [float(f"{t:.6g}") for t in g["snap_t"]] == rec
