@@ -37,6 +37,7 @@ typedef struct mh_grid_s* mh_grid;
 typedef struct mh_stepper_s* mh_stepper;
 typedef struct mh_batch_s* mh_batch;
 typedef struct mh_adjoint_s* mh_adjoint;
+typedef struct mh_snapshot_s* mh_snapshot;
 typedef void* mh_stream; /* hipStream_t */
 
 /* Status codes (the reference prints and calls std::exit instead:
@@ -144,6 +145,11 @@ int mh_stepper_advance(mh_stepper s, int nsteps, mh_stream stream);
 int mh_stepper_clock(mh_stepper s, double* t, int* steps, double* dt_log, mh_stream stream);
 /* Re-fills the ghosts of the state (after the last step). */
 int mh_stepper_finish(mh_stepper s, mh_stream stream);
+/* Holds the loop after the first step with t >= t_stop (0: no stop) -- the
+ * reference writes a snapshot there (scripts/gpu/main_gpu.cu:272-298); steps
+ * enqueued past it are no-ops until the next call moves the stop and
+ * releases the clock.  Enqueued on `stream`. */
+int mh_stepper_set_stop(mh_stepper s, double t_stop, mh_stream stream);
 /* mh_stepper_advance with a HIP event pair around every launch; synchronises.
  * ms_total: elapsed time of the n steps; ms_kernels[3]: average duration per
  * step of the clock kernel, the x sweep and the y sweep.  (No reference
@@ -167,6 +173,35 @@ typedef struct {
 } mh_launch_shape;
 int mh_set_launch_shape(const mh_launch_shape* shape);
 int mh_get_launch_shape(mh_launch_shape* shape);
+
+/* ---- snapshot output: replaces the download + write_aos_csv of the
+ *      reference's loop (scripts/gpu/main_gpu.cu:272-298, :146-171;
+ *      Grid2DGPU::download_to_aos, head/gpu/grid_gpu.cuh:106-121).  A capture
+ *      copies the state on the device (on the caller's stream) and then to
+ *      pinned host memory on the snapshot's own copy stream, so stepping
+ *      continues while the data travels; the CSV writer formats rows on
+ *      several host threads. ---------------------------------------------- */
+/* nbuf slots (1..8), each a device copy and pinned host planes of a grid
+ * shaped like `like`. */
+int mh_snapshot_create(mh_grid like, int nbuf, mh_snapshot* out);
+int mh_snapshot_destroy(mh_snapshot s);
+/* Enqueues the capture of U into `slot` (waits on the device for that
+ * slot's previous host copy).  The caller must be done with the slot's host
+ * planes from an earlier capture. */
+int mh_snapshot_capture(mh_snapshot s, mh_grid U, int slot, mh_stream stream);
+/* Waits for the slot's host copy; planes (may be NULL) receives its four
+ * host planes of (ny+4)*(nx+4) doubles in the reference's cell order
+ * (index j*(nx+4)+i, ghosts included). */
+int mh_snapshot_wait(mh_snapshot s, int slot, double** planes);
+/* Waits for the slot, then writes it as the reference's snapshot CSV
+ * (header i,j,x,y,rho,rhou,rhov,E; one row per interior cell; i, j with the
+ * ghost offset; values as %.{precision}g, the reference's ostream default
+ * being 6; scripts/gpu/main_gpu.cu:146-171) on nthreads host threads
+ * (0: up to 16). */
+int mh_snapshot_write_csv(mh_snapshot s, int slot, const char* path, int precision, int nthreads);
+/* The same CSV from a host state in the reference AoS layout
+ * ((ny+4)*(nx+4) Conserved cells) of the grid `desc` describes. */
+int mh_write_csv_aos(const double* host_aos, const mh_grid_desc* desc, int precision, int nthreads, const char* path);
 
 /* ---- initial conditions on the device (point sampling at cell centres,
  *      ghosts included: src/init.cpp:29-38).  kind 0: reference shock_bubble

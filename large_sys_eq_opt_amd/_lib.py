@@ -63,6 +63,7 @@ SIGNATURES = {
     "mh_stepper_advance": ([_P, _I, _P], _I),
     "mh_stepper_clock": ([_P, C.POINTER(_D), C.POINTER(_I), _P, _P], _I),
     "mh_stepper_finish": ([_P, _P], _I),
+    "mh_stepper_set_stop": ([_P, _D, _P], _I),
     "mh_init_state": ([_P, _I, _P, _P], _I),
     "mh_stepper_advance_profiled": ([_P, _I, _P, C.POINTER(_D), C.POINTER(_D)], _I),
     "mh_set_launch_shape": ([C.POINTER(LaunchShape)], _I),
@@ -83,6 +84,12 @@ SIGNATURES = {
     "mh_adjoint_forward": ([_P, _P, _I, _D, _P, _P, _P, _P], _I),
     "mh_adjoint_backward": ([_P, _P, _P, _P], _I),
     "mh_adjoint_dt": ([_P, _P, C.POINTER(_I)], _I),
+    "mh_snapshot_create": ([_P, _I, C.POINTER(_P)], _I),
+    "mh_snapshot_destroy": ([_P], _I),
+    "mh_snapshot_capture": ([_P, _P, _I, _P], _I),
+    "mh_snapshot_wait": ([_P, _I, _P * 4], _I),
+    "mh_snapshot_write_csv": ([_P, _I, C.c_char_p, _I, _I], _I),
+    "mh_write_csv_aos": ([_P, C.POINTER(GridDesc), _I, _I, C.c_char_p], _I),
     "mh_grid_zero": ([_P, _P], _I),
     "mh_grid_copy": ([_P, _P, _P], _I),
     "mh_copy": ([_P, _P, _I64, _P], _I),

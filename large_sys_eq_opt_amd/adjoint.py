@@ -30,7 +30,7 @@ import numpy as np
 from . import _lib
 from . import cases as _cases
 from ._lib import GridDesc, call
-from .solver import Boundary, Grid2DGPU, _bc_ptr, _stream
+from .solver import (Boundary, Grid2DGPU, _bc_ptr, _stream, advance_second_order_gpu, apply_boundary_gpu)
 
 
 class Adjoint:
@@ -41,7 +41,12 @@ class Adjoint:
         self.bc = bc
         call("mh_adjoint_create", C.byref(desc), int(solver), _bc_ptr(bc), int(checkpoint_every), int(max_steps),
              C.byref(self._h))
+        self.solver = int(solver)
         self.nsteps = 0
+        # Each forward() replaces the stored trajectory (checkpoints and dt
+        # sequence) that backward() reverses; callers that keep a trajectory
+        # for later note this counter and check it before backward().
+        self.generation = 0
 
     def step_vjp(self, U: Grid2DGPU, dt: float, lam_in: Grid2DGPU, lam_out: Grid2DGPU, stream=None):
         call("mh_adjoint_step_vjp", self._h, U.handle, float(dt), lam_in.handle, lam_out.handle, _stream(stream))
@@ -57,6 +62,7 @@ class Adjoint:
              None if din is None else din.ctypes.data_as(C.c_void_p), dts.ctypes.data_as(C.c_void_p),
              None if UN is None else UN.handle, _stream(stream))
         self.nsteps = int(nsteps)
+        self.generation += 1
         return dts[:nsteps].copy()
 
     def backward(self, lamN: Grid2DGPU, lam0: Grid2DGPU, stream=None):
@@ -176,25 +182,49 @@ class ShockBubbleAdjoint:
         self.lamN.upload_from_aos(self.weights)
         self.dts = None
         self._dU = None
+        self._gen = -1
 
     def objective(self, rho_b, R, dt_in=None):
-        """Forward run from the smoothed state; returns J and keeps the
-        trajectory's checkpoints for gradient().  dt_in freezes the time
-        steps (default: CFL steps, recorded in self.dts)."""
+        """Forward run from the smoothed state; returns J.  Without dt_in the
+        run takes CFL steps (recorded in self.dts) through the checkpointing
+        forward, and gradient() reverses it.  With dt_in (frozen time steps,
+        e.g. for finite differences) it is a plain forward run that leaves
+        the stored trajectory alone."""
         U, dUr, dUR = smooth_bubble_state(self.cfg, rho_b, R, self.params)
-        self.U0.upload_from_aos(U)
-        dts = self.adj.forward(self.U0, self.nsteps, self.cfg.cfl, dt_in, self.UN)
         if dt_in is None:
-            self.dts = dts
+            self.U0.upload_from_aos(U)
+            self.dts = self.adj.forward(self.U0, self.nsteps, self.cfg.cfl, None, self.UN)
             self._dU = (dUr, dUR)
-        return float(np.sum(self.weights[..., 0] * self.UN.download_to_aos()[..., 0]))
+            self._gen = self.adj.generation
+            return float(np.sum(self.weights[..., 0] * self.UN.download_to_aos()[..., 0]))
+        dt_in = np.asarray(dt_in, dtype=np.float64)
+        if dt_in.shape != (self.nsteps,):
+            raise ValueError(f"dt_in must hold {self.nsteps} time steps")
+        g = Grid2DGPU.from_config(self.cfg)
+        tmp, nxt = Grid2DGPU.from_config(self.cfg), Grid2DGPU.from_config(self.cfg)
+        try:
+            g.upload_from_aos(U)
+            apply_boundary_gpu(g, self.bc)
+            for dt in dt_in:
+                advance_second_order_gpu(g, tmp, nxt, None, float(dt), self.adj.solver, self.bc)
+                g.swap(nxt)
+            UN = g.download_to_aos()
+        finally:
+            for x in (g, tmp, nxt):
+                x.close()
+        return float(np.sum(self.weights[..., 0] * UN[..., 0]))
 
     def final_state(self):
         return self.UN.download_to_aos()
 
     def gradient(self):
         """(dJ/drho_b (ny, nx), dJ/dR) at the last objective() call made
-        without dt_in."""
+        without dt_in.  Raises if the adjoint workspace ran another forward
+        since (its stored trajectory is then not that call's)."""
+        if self._dU is None:
+            raise RuntimeError("gradient(): call objective() without dt_in first")
+        if self.adj.generation != self._gen:
+            raise RuntimeError("gradient(): the adjoint workspace has run another forward since objective()")
         self.adj.backward(self.lamN, self.lam0)
         lam0 = self.lam0.download_to_aos()[2:-2, 2:-2]
         dUr, dUR = self._dU
@@ -249,7 +279,9 @@ class EulerSolve:
 
     EulerSolve.apply(U0, adj, nsteps, cfl, dt_in) -> UN, where U0 and UN are
     (4, ny, nx) float64 CUDA tensors of interior conserved states.  The dt
-    sequence of the forward call is kept on ``adj``.
+    sequence of the forward call is kept on ``adj``.  One Adjoint holds one
+    trajectory: a second apply() on the same Adjoint before the first one's
+    backward makes that backward raise instead of reversing the wrong run.
     """
 
     _fn = None
@@ -273,6 +305,7 @@ class EulerSolve:
                     gN.copy_interior_to_tensor(UN, stream)
                     torch.cuda.synchronize()
                     ctx.adj = adj
+                    ctx.generation = adj.generation
                     g0.close()
                     gN.close()
                     return UN
@@ -280,6 +313,10 @@ class EulerSolve:
                 @staticmethod
                 def backward(ctx, gUN):
                     adj = ctx.adj
+                    if adj.generation != ctx.generation:
+                        raise RuntimeError("EulerSolve.backward: the Adjoint workspace ran another forward after this "
+                                           "one; its stored trajectory is not this call's (use one Adjoint per "
+                                           "pending backward)")
                     cfg = adj.cfg
                     gUN = gUN.detach().contiguous()
                     lamN, lam0 = Grid2DGPU.from_config(cfg), Grid2DGPU.from_config(cfg)

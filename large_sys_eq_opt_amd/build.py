@@ -16,7 +16,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 INCLUDE = os.path.join(os.path.dirname(HERE), "include")
 LIB = os.path.join(HERE, "libmh_euler.so")
-SOURCES = ["solver.hip", "batch.hip", "adjoint.hip"]
+SOURCES = ["solver.hip", "batch.hip", "adjoint.hip", "snapshot.hip"]
 ARCH = os.environ.get("MH_OFFLOAD_ARCH", "gfx950")
 
 # -ffp-contract=off: no fused multiply-add may change a rounding, so the

@@ -11,7 +11,8 @@ first steps with t >= k t_end / 5; the defaults are the shock-bubble case and
 the HLL solver.  Errors print "[GPU] Error: ..." and exit with status 1.
 
 Options with no reference counterpart: --device-loop keeps dt and t on the
-device (no per-step host round trip; no intermediate snapshots),
+device (no per-step host round trip; the clock holds at each snapshot time
+and the snapshot travels to the host while the next steps run),
 --precision N writes N significant digits (17: round-trip exact), --outdir
 DIR writes the snapshots there.
 """
@@ -82,8 +83,6 @@ def parse_run_options(argv) -> RunOptions:
         i += 1
     if opts.resolution_scale <= 0:
         raise ValueError("resolution_scale must be positive.")
-    if opts.device_loop and opts.write_output:
-        raise ValueError("--device-loop takes no intermediate snapshots; drop --output.")
     return opts
 
 

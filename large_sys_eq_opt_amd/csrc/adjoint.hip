@@ -49,6 +49,7 @@ namespace mh {
 int launch_sweep_host(const mh_grid_s* in, const mh_grid_s* out, int dir, int solver, double dt, const Bc& bc,
                       int ghosts, hipStream_t st);
 int max_speed_host(const mh_grid_s* g, double* smax, hipStream_t st);
+int stepper_start(mh_stepper s, double cfl, hipStream_t st);
 
 // Output rows -2..ny+1 are cut into nch chunks of seg rows, the last one
 // running to row ny+1 (seg..2*seg-1 rows), so that rows -2..0 share the
@@ -212,6 +213,10 @@ struct mh_adjoint_s {
     hipEvent_t rec_done[2] = {nullptr, nullptr}, vjp_done[2] = {nullptr, nullptr};
     std::vector<double> dts;
     int nsteps = 0;
+    // device-resident loop over `work` for forward runs with CFL time steps
+    // (dt from the speed maximum fused into the y sweep: no host round trip
+    // per step)
+    mh_stepper stepper = nullptr;
 };
 
 namespace {
@@ -370,6 +375,7 @@ int mh_adjoint_create(const mh_grid_desc* desc, int solver, const mh_bc* bc, int
         rc = check_hip(hipEventCreateWithFlags(&a->rec_done[b], hipEventDisableTiming), "hipEventCreate");
         if (!rc) rc = check_hip(hipEventCreateWithFlags(&a->vjp_done[b], hipEventDisableTiming), "hipEventCreate");
     }
+    if (!rc) rc = mh_stepper_create(a->work, solver, 1.0, 0.0, bc, std::max(max_steps, 1), &a->stepper, nullptr);
     // Cells no kernel writes (the ghost rows of the ping-pong grids) read
     // as zero.
     for (mh_grid_s* g : {a->bTt, a->A0, a->A1})
@@ -383,6 +389,7 @@ int mh_adjoint_create(const mh_grid_desc* desc, int solver, const mh_bc* bc, int
 int mh_adjoint_destroy(mh_adjoint a) {
     if (!a) return MH_OK;
     if (a->aux) (void)hipStreamSynchronize(a->aux);
+    if (a->stepper) mh_stepper_destroy(a->stepper);
     for (mh_grid_s* g : {a->T, a->Ut, a->bTt, a->work, a->A0, a->A1})
         if (g) mh_grid_destroy(g);
     for (mh_grid_s* g : a->ckpt) if (g) mh_grid_destroy(g);
@@ -428,29 +435,40 @@ int mh_adjoint_forward(mh_adjoint a, mh_grid U0, int nsteps, double cfl, const d
                        mh_grid UN, mh_stream stream) {
     if (!a || !U0 || nsteps < 0 || nsteps > a->max_steps) return fail(MH_ERR_INVALID, "mh_adjoint_forward: bad argument");
     if (!same(U0, a->d) || (UN && !same(UN, a->d))) return fail(MH_ERR_INVALID, "mh_adjoint_forward: shape mismatch");
+    if (!dt_in && !(cfl > 0.0)) return fail(MH_ERR_INVALID, "mh_adjoint_forward: cfl must be positive");
     hipStream_t st = (hipStream_t)stream;
-    a->dts.assign(nsteps, 0.0);
+    std::vector<double> dts(nsteps, 0.0);
+    // Each step is the x sweep work -> T and the y sweep T -> work; their
+    // fused ghost writes supply every ghost cell the next sweep reads (T's
+    // y-ghost rows, work's x-ghost columns), which are also all the
+    // checkpoints' recompute and the products read.
     int rc = launch_copy(U0, a->work, st);
-    for (int n = 0; n < nsteps && !rc; ++n) {
-        rc = launch_fill(a->work, a->bc, st);
-        if (!rc && n % a->K == 0) rc = launch_copy(a->work, a->ckpt[n / a->K], st);
-        double dt = 0.0;
-        if (!rc) {
-            if (dt_in) {
-                dt = dt_in[n];
-            } else {
-                double m = 0.0;
-                rc = max_speed_host(a->work, &m, st);
-                if (!rc && !(m > 0.0)) rc = fail(MH_ERR_WAVESPEED, "adjoint forward: non-positive maximum wave speed");
-                if (!rc) dt = cfl * std::min(a->work->dx, a->work->dy) / m;
-            }
+    if (dt_in) {
+        if (!rc) rc = launch_fill(a->work, a->bc, st);
+        for (int n = 0; n < nsteps && !rc; ++n) {
+            if (n % a->K == 0) rc = launch_copy(a->work, a->ckpt[n / a->K], st);
+            if (!rc) rc = launch_sweep_host(a->work, a->T, kX, a->solver, dt_in[n], a->bc, 1, st);
+            if (!rc) rc = launch_sweep_host(a->T, a->work, kY, a->solver, dt_in[n], a->bc, 1, st);
+            dts[n] = dt_in[n];
         }
-        if (!rc) rc = mh_advance_second_order(a->work, a->T, a->work, dt, a->solver, &a->bc_c, stream);
-        a->dts[n] = dt;
+    } else {
+        // dt = cfl*min(dx,dy)/max speed on the device (the arithmetic of
+        // mh_compute_dt), logged per step and read back once at the end
+        if (!rc) rc = stepper_start(a->stepper, cfl, st);
+        for (int n = 0; n < nsteps && !rc; ++n) {
+            if (n % a->K == 0) rc = launch_copy(a->work, a->ckpt[n / a->K], st);
+            if (!rc) rc = mh_stepper_advance(a->stepper, 1, stream);
+        }
+        double t = 0.0;
+        int done = 0;
+        if (!rc) rc = mh_stepper_clock(a->stepper, &t, &done, dts.data(), stream);
+        if (!rc && done != nsteps) rc = fail(MH_ERR_LAUNCH, "mh_adjoint_forward: the device loop stopped early");
     }
+    if (!rc && UN) rc = launch_fill(a->work, a->bc, st);
     if (!rc && UN) rc = launch_copy(a->work, UN, st);
     if (!rc) rc = check_hip(hipStreamSynchronize(st), "hipStreamSynchronize");
     if (rc) return rc;
+    a->dts = dts;
     a->nsteps = nsteps;
     if (dt_out) std::memcpy(dt_out, a->dts.data(), sizeof(double) * nsteps);
     return MH_OK;

@@ -87,22 +87,29 @@ __device__ __forceinline__ bool inst_dt(const BatchInst& I, const BatchClock& c,
     return true;
 }
 
-__device__ void grid_barrier(unsigned* bar, unsigned nblocks, int* err) {
+// Returns false (in every thread of the workgroup) when the bounded spin
+// gave up; the flag in *err then stops this and every later launch.
+__device__ bool grid_barrier(unsigned* bar, unsigned nblocks, int* err) {
+    __shared__ int ok;
     __syncthreads();
     if (threadIdx.x == 0) {
         __threadfence();
         atomicAdd(bar, 1u);
         long spins = 0;
+        int good = 1;
         while (__hip_atomic_load(bar, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < nblocks) {
             __builtin_amdgcn_s_sleep(8);
             if (++spins > (1L << 24)) {   // ~ seconds: not all workgroups resident
                 atomicExch(err, 1);
+                good = 0;
                 break;
             }
         }
         __threadfence();
+        ok = good && __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0;
     }
     __syncthreads();
+    return ok != 0;
 }
 
 // Clock commit of phase x, shared by the persistent and dispatched kernels:
@@ -138,6 +145,11 @@ __global__ __launch_bounds__(256) void k_batch_step(BatchArgs a) {
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     const int gw = blockIdx.x * 4 + w;
+    // A barrier that gave up in an earlier launch leaves the error flag set:
+    // this launch then touches nothing, so the instances keep the state of
+    // the last complete step (the clocks of the failed step stay committed;
+    // mh_batch_clock reports the error and mh_batch_prepare clears it).
+    if (__hip_atomic_load(&a.ctrl->err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
     const int step = __hip_atomic_load(&a.ctrl->step, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     const int par = step & 1;
     if (a.phases & 1) {
@@ -151,7 +163,8 @@ __global__ __launch_bounds__(256) void k_batch_step(BatchArgs a) {
             for (int j = t.j0; j < t.j1; ++j) x_tile<S, 2>(I.U, I.T, j, t.i0, t.i1, dtd, 0.5 * dtd, I.bc, 1, X[w].x, lane);
         }
     }
-    if (a.phases == 3) grid_barrier(&a.ctrl->bar[par], gridDim.x, &a.ctrl->err);
+    // phase y rewrites U: never on top of an incomplete x phase
+    if (a.phases == 3 && !grid_barrier(&a.ctrl->bar[par], gridDim.x, &a.ctrl->err)) return;
     if (a.phases & 2) {
         for (int q = a.ystart[gw]; q < a.ystart[gw + 1]; ++q) {
             const YTile t = a.yt[q];

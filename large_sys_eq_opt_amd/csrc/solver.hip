@@ -27,6 +27,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -55,11 +56,13 @@ int check_hip(hipError_t e, const char* what) {
 // ---------------------------------------------------------------------------
 // device clock and sweep kernels
 // ---------------------------------------------------------------------------
-// Device clock of the device-resident time loop.
+// Device clock of the device-resident time loop.  t_stop > 0 holds the
+// loop after the first step with t >= t_stop (a snapshot point): the clock
+// sets done and stopped, and mh_stepper_set_stop releases it.
 struct Clock {
-    double t, dt, t_end, cfl, h;
+    double t, dt, t_end, cfl, h, t_stop;
     unsigned long long smax[2];   // bit patterns of non-negative doubles
-    int step, slot, pending, done, status, log_cap;
+    int step, slot, pending, done, status, log_cap, stopped;
 };
 
 // dt source of a sweep: a host value or the device clock.
@@ -261,6 +264,11 @@ __global__ void k_clock(Clock* c, double* dt_log, int commit_only) {
         c->done = 1;
         return;
     }
+    if (c->t_stop > 0.0 && !(c->t < c->t_stop)) {
+        c->done = 1;
+        c->stopped = 1;
+        return;
+    }
     const double m = __longlong_as_double((long long)c->smax[c->slot]);
     if (!(m > 0.0)) {
         c->done = 1;
@@ -273,6 +281,36 @@ __global__ void k_clock(Clock* c, double* dt_log, int commit_only) {
     c->smax[c->slot ^ 1] = 0ull;
     if (dt_log && c->step < c->log_cap) dt_log[c->step] = dt;
     c->pending = 1;
+}
+
+// Starts a clock at t = 0 with both speed slots clear.
+__global__ void k_clock_init(Clock* c, double t_end, double cfl, double h, int log_cap) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    c->t = 0.0;
+    c->dt = 0.0;
+    c->t_end = t_end;
+    c->cfl = cfl;
+    c->h = h;
+    c->t_stop = 0.0;
+    c->smax[0] = 0ull;
+    c->smax[1] = 0ull;
+    c->step = 0;
+    c->slot = 0;
+    c->pending = 0;
+    c->done = 0;
+    c->status = 0;
+    c->log_cap = log_cap;
+    c->stopped = 0;
+}
+
+// Moves the clock's stop time; releases a clock held at the previous one.
+__global__ void k_clock_stop(Clock* c, double t_stop) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    c->t_stop = t_stop;
+    if (c->stopped) {
+        c->stopped = 0;
+        c->done = 0;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -434,7 +472,16 @@ int valid_bc(const mh_bc* b) {
 // wavefront per workgroup, and four y tiles per workgroup at the compiler's
 // register choice measured fastest for HLLC at 8192^2 on MI355X; the
 // autotuner can override every field.
+// The shape is process-wide: mh_set_launch_shape replaces it under a lock
+// and every launch reads one consistent copy (shape()), so a host thread
+// that sets it while another launches races on nothing.
 static mh_launch_shape g_shape = {1020, 128, 1, 2, 1, 4, 8192};
+static std::mutex g_shape_lock;
+
+mh_launch_shape shape() {
+    std::lock_guard<std::mutex> lk(g_shape_lock);
+    return g_shape;
+}
 
 template <int S, int L, int WPB>
 void go_x(dim3 g, hipStream_t st, const View& vi, const View& vo, const SweepArgs& a) {
@@ -469,9 +516,9 @@ void go_y_s(int waves, int wpb, dim3 g, hipStream_t st, const View& vi, const Vi
 // they reread more: at 4096^2 the step fell from 0.65 to 0.58 ms (2048
 // tiles -> 8192), at 8192^2 nothing changes (the default shape gives 8192 y
 // tiles).  A grid left below min_tiles takes one tile per workgroup.
-int seg_x_for(const mh_grid_s* g) {
-    const int nx = g->d.nx, ny = g->d.ny, want_tiles = g_shape.min_tiles;
-    int seg = g_shape.seg_x > 0 ? g_shape.seg_x : nx;
+int seg_x_for(const mh_grid_s* g, const mh_launch_shape& sh) {
+    const int nx = g->d.nx, ny = g->d.ny, want_tiles = sh.min_tiles;
+    int seg = sh.seg_x > 0 ? sh.seg_x : nx;
     if ((int64_t)((nx + seg - 1) / seg) * ny < want_tiles && seg > 60) {
         const int per_row = (want_tiles + ny - 1) / ny;
         const int want = (nx + per_row - 1) / per_row;
@@ -480,9 +527,9 @@ int seg_x_for(const mh_grid_s* g) {
     return seg;
 }
 
-int seg_y_for(const mh_grid_s* g) {
-    const int nx = g->d.nx, ny = g->d.ny, want_tiles = g_shape.min_tiles;
-    int seg = g_shape.seg_y > 0 ? g_shape.seg_y : ny;
+int seg_y_for(const mh_grid_s* g, const mh_launch_shape& sh) {
+    const int nx = g->d.nx, ny = g->d.ny, want_tiles = sh.min_tiles;
+    int seg = sh.seg_y > 0 ? sh.seg_y : ny;
     const int ngroups = (nx + 63) / 64;
     if ((int64_t)ngroups * ((ny + seg - 1) / seg) < want_tiles && seg > 4) {
         const int chunks = (want_tiles + ngroups - 1) / ngroups;
@@ -501,14 +548,15 @@ int launch_sweep(const mh_grid_s* in, const mh_grid_s* out, int dir, int solver,
     a.clock_speed = clock_speed;
     a.redo = nullptr;
     const View vi = in->view(), vo = out->view();
+    const mh_launch_shape sh = shape();
     if (dir == kX) {
         a.d = in->dx;
-        a.seg = seg_x_for(in);
+        a.seg = seg_x_for(in, sh);
         a.ngroups = (in->d.nx + a.seg - 1) / a.seg;
         a.ntiles = a.ngroups * in->d.ny;
-        const int wpb = a.ntiles < g_shape.min_tiles ? 1 : g_shape.x_wpb;
+        const int wpb = a.ntiles < sh.min_tiles ? 1 : sh.x_wpb;
         const dim3 g((unsigned)((a.ntiles + wpb - 1) / wpb));
-        const int L = g_shape.x_layout;
+        const int L = sh.x_layout;
         switch (solver) {
             case kHLL: go_x_s<kHLL>(L, wpb, g, st, vi, vo, a); break;
             case kHLLC: go_x_s<kHLLC>(L, wpb, g, st, vi, vo, a); break;
@@ -517,16 +565,16 @@ int launch_sweep(const mh_grid_s* in, const mh_grid_s* out, int dir, int solver,
         }
     } else {
         a.d = in->dy;
-        a.seg = seg_y_for(in);
+        a.seg = seg_y_for(in, sh);
         a.ngroups = (in->d.nx + 63) / 64;
         a.ntiles = a.ngroups * ((in->d.ny + a.seg - 1) / a.seg);
         // The redo list costs a second launch: worth it only where the sweep
         // is throughput-bound (at least min_tiles tiles); small, latency-bound
         // grids take the IEEE pass in place.
-        if (out->redo && a.ntiles >= g_shape.min_tiles && a.ntiles <= out->redo_cap) a.redo = out->redo;
-        const int wpb = a.ntiles < g_shape.min_tiles ? 1 : g_shape.y_wpb;
+        if (out->redo && a.ntiles >= sh.min_tiles && a.ntiles <= out->redo_cap) a.redo = out->redo;
+        const int wpb = a.ntiles < sh.min_tiles ? 1 : sh.y_wpb;
         const dim3 g((unsigned)((a.ntiles + wpb - 1) / wpb));
-        const int W = g_shape.y_waves;
+        const int W = sh.y_waves;
         switch (solver) {
             case kHLL: go_y_s<kHLL>(W, wpb, g, st, vi, vo, a); break;
             case kHLLC: go_y_s<kHLLC>(W, wpb, g, st, vi, vo, a); break;
@@ -580,10 +628,26 @@ struct mh_stepper_s {
     mh_grid_s* tmp;
     int solver;
     Bc bc;
+    double cfl, t_end;
     Clock* clock;      // device
     double* dt_log;    // device
     int log_cap;
 };
+
+namespace mh {
+// (Re)starts a stepper at t = 0 from the current state of its grid: fills
+// the ghosts and reduces the state's max signal speed into the clock.
+// Enqueues only.  Used by mh_stepper_create and the adjoint's forward run.
+int stepper_start(mh_stepper s, double cfl, hipStream_t st) {
+    s->cfl = cfl;
+    hipLaunchKernelGGL(k_clock_init, dim3(1), dim3(64), 0, st, s->clock, s->t_end, cfl,
+                       std::min(s->U->dx, s->U->dy), s->log_cap);
+    int rc = check_hip(hipGetLastError(), "clock launch");
+    if (!rc) rc = launch_fill(s->U, s->bc, st);
+    if (!rc) rc = launch_max_speed(s->U, &s->clock->smax[0], st);
+    return rc;
+}
+}  // namespace mh
 
 extern "C" {
 
@@ -822,20 +886,14 @@ int mh_stepper_create(mh_grid U, int solver, double cfl, double t_end, const mh_
     s->U = U;
     s->solver = solver;
     s->bc = to_bc(bc);
+    s->t_end = t_end;
     s->log_cap = dt_log_capacity > 0 ? dt_log_capacity : 0;
     int rc = mh_grid_create(&U->d, &s->tmp);
     if (rc) { delete s; return rc; }
     rc = check_hip(hipMalloc((void**)&s->clock, sizeof(Clock)), "hipMalloc(clock)");
     s->dt_log = nullptr;
     if (!rc && s->log_cap) rc = check_hip(hipMalloc((void**)&s->dt_log, sizeof(double) * s->log_cap), "hipMalloc(dt log)");
-    if (rc) { mh_stepper_destroy(s); return rc; }
-    Clock c{};
-    c.t = 0.0; c.dt = 0.0; c.t_end = t_end; c.cfl = cfl; c.h = std::min(U->dx, U->dy);
-    c.smax[0] = c.smax[1] = 0ull;
-    c.step = 0; c.slot = 0; c.pending = 0; c.done = 0; c.status = 0; c.log_cap = s->log_cap;
-    rc = check_hip(hipMemcpyAsync(s->clock, &c, sizeof(c), hipMemcpyHostToDevice, st), "hipMemcpyAsync(clock)");
-    if (!rc) rc = launch_fill(U, s->bc, st);
-    if (!rc) rc = launch_max_speed(U, &s->clock->smax[0], st);
+    if (!rc) rc = stepper_start(s, cfl, st);
     if (!rc) rc = check_hip(hipStreamSynchronize(st), "hipStreamSynchronize");
     if (rc) { mh_stepper_destroy(s); return rc; }
     *out = s;
@@ -874,16 +932,26 @@ int mh_stepper_clock(mh_stepper s, double* t, int* steps, double* dt_log, mh_str
     Clock c;
     int rc = check_hip(hipGetLastError(), "clock launch");
     if (!rc) rc = check_hip(hipMemcpyAsync(&c, s->clock, sizeof(c), hipMemcpyDeviceToHost, st), "hipMemcpyAsync");
-    if (!rc && dt_log && s->log_cap)
-        rc = check_hip(hipMemcpyAsync(dt_log, s->dt_log, sizeof(double) * std::min(s->log_cap, std::max(c.step, 0)),
+    // c.step sizes the log copy: wait for the clock first
+    if (!rc) rc = check_hip(hipStreamSynchronize(st), "hipStreamSynchronize");
+    if (!rc && dt_log && s->log_cap && c.step > 0) {
+        rc = check_hip(hipMemcpyAsync(dt_log, s->dt_log, sizeof(double) * std::min(s->log_cap, c.step),
                                       hipMemcpyDeviceToHost, st),
                        "hipMemcpyAsync");
-    if (!rc) rc = check_hip(hipStreamSynchronize(st), "hipStreamSynchronize");
+        if (!rc) rc = check_hip(hipStreamSynchronize(st), "hipStreamSynchronize");
+    }
     if (rc) return rc;
     if (t) *t = c.t;
     if (steps) *steps = c.step;
     if (c.status) return fail(c.status, "stepper: non-positive maximum wave speed");
     return MH_OK;
+}
+
+int mh_stepper_set_stop(mh_stepper s, double t_stop, mh_stream stream) {
+    if (!s) return fail(MH_ERR_INVALID, "mh_stepper_set_stop: null stepper");
+    if (!(t_stop >= 0.0)) return fail(MH_ERR_INVALID, "mh_stepper_set_stop: t_stop must be >= 0");
+    hipLaunchKernelGGL(k_clock_stop, dim3(1), dim3(64), 0, (hipStream_t)stream, s->clock, t_stop);
+    return check_hip(hipGetLastError(), "clock launch");
 }
 
 int mh_stepper_finish(mh_stepper s, mh_stream stream) {
@@ -1074,13 +1142,14 @@ int mh_set_launch_shape(const mh_launch_shape* s) {
     if (s->x_layout < 0 || s->x_layout > 2) return fail(MH_ERR_INVALID, "mh_set_launch_shape: x_layout must be 0, 1 or 2");
     if ((s->x_wpb != 1 && s->x_wpb != 4) || (s->y_wpb != 1 && s->y_wpb != 4))
         return fail(MH_ERR_INVALID, "mh_set_launch_shape: wavefronts per workgroup must be 1 or 4");
+    std::lock_guard<std::mutex> lk(g_shape_lock);
     g_shape = *s;
     return MH_OK;
 }
 
 int mh_get_launch_shape(mh_launch_shape* s) {
     if (!s) return fail(MH_ERR_INVALID, "mh_get_launch_shape: null shape");
-    *s = g_shape;
+    *s = shape();
     return MH_OK;
 }
 

@@ -298,6 +298,11 @@ class Stepper:
     def finish(self, stream=None):
         call("mh_stepper_finish", self._h, _stream(stream))
 
+    def set_stop(self, t_stop: float, stream=None):
+        """Holds the loop after the first step with t >= t_stop (a snapshot
+        point); enqueued steps past it do nothing until the next set_stop."""
+        call("mh_stepper_set_stop", self._h, float(t_stop), _stream(stream))
+
     def close(self):
         if self._h:
             _lib.load().mh_stepper_destroy(self._h)
@@ -308,3 +313,57 @@ class Stepper:
             self.close()
         except Exception:
             pass
+
+
+class Snapshot:
+    """Asynchronous snapshot staging for grids shaped like ``like``: each of
+    ``nbuf`` slots holds a device copy and pinned host planes.  capture()
+    enqueues the device copy on the caller's stream and the host copy on the
+    snapshot's own stream; wait()/aos()/write_csv() wait for that slot."""
+
+    def __init__(self, like: Grid2DGPU, nbuf: int = 2):
+        self._h = C.c_void_p()
+        self.nx, self.ny, self.nbuf = like.nx, like.ny, int(nbuf)
+        call("mh_snapshot_create", like.handle, self.nbuf, C.byref(self._h))
+
+    def capture(self, U: Grid2DGPU, slot: int, stream=None):
+        call("mh_snapshot_capture", self._h, U.handle, int(slot), _stream(stream))
+
+    def planes(self, slot: int):
+        """The slot's four host planes as (ny+4, nx+4) numpy views (valid until
+        the slot is captured again)."""
+        p = (C.c_void_p * 4)()
+        call("mh_snapshot_wait", self._h, int(slot), p)
+        shape = (self.ny + 4, self.nx + 4)
+        n = shape[0] * shape[1]
+        return [np.ctypeslib.as_array(C.cast(p[c], C.POINTER(C.c_double)), shape=(n,)).reshape(shape)
+                for c in range(4)]
+
+    def aos(self, slot: int) -> np.ndarray:
+        """A copy of the slot in the reference AoS layout, (ny+4, nx+4, 4)."""
+        return np.stack(self.planes(slot), axis=-1)
+
+    def write_csv(self, slot: int, path: str, precision: int = 6, nthreads: int = 0):
+        call("mh_snapshot_write_csv", self._h, int(slot), path.encode(), int(precision), int(nthreads))
+
+    def close(self):
+        if self._h:
+            _lib.load().mh_snapshot_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def write_csv_aos(U_aos: np.ndarray, cfg, filename: str, precision: int = 6, nthreads: int = 0):
+    """The reference's snapshot CSV (scripts/cpu/main_cpu.cpp:136-164) of a host
+    AoS state, formatted by the library on several threads."""
+    a = np.ascontiguousarray(U_aos, dtype=np.float64)
+    if a.shape != (cfg.ny + 4, cfg.nx + 4, 4):
+        raise ValueError(f"expected shape {(cfg.ny + 4, cfg.nx + 4, 4)}, got {a.shape}")
+    desc = GridDesc(int(cfg.nx), int(cfg.ny), float(cfg.x_min), float(cfg.x_max), float(cfg.y_min), float(cfg.y_max))
+    call("mh_write_csv_aos", a.ctypes.data_as(C.c_void_p), C.byref(desc), int(precision), int(nthreads),
+         filename.encode())

@@ -2,9 +2,9 @@
 from __future__ import annotations
 
 import numpy as np
+import pytest
 
 from conftest import same_bits
-import pytest
 
 from large_sys_eq_opt_amd import cases
 from large_sys_eq_opt_amd.driver import read_grid_csv, snapshot_name, write_grid_csv
@@ -70,6 +70,41 @@ def test_snapshot_csv_format_and_roundtrip(tmp_path):
     write_grid_csv(U, cfg, exact, precision=17)
     assert same_bits(read_grid_csv(exact, 6, 4), U[2:-2, 2:-2])
     assert snapshot_name("gpu", "blast_wave", "hllc", 2, 3) == "outputs/gpu_blast_wave_hllc_n2_snapshot_3.csv"
+
+
+def _csv_rows_python(U, cfg, precision):
+    """The snapshot rows as the reference's ostream prints them
+    (scripts/cpu/main_cpu.cpp:136-164), formatted one cell at a time."""
+    dx = (cfg.x_max - cfg.x_min) / float(cfg.nx)
+    dy = (cfg.y_max - cfg.y_min) / float(cfg.ny)
+    g = lambda v: format(float(v), f".{precision}g")  # noqa: E731
+    rows = ["i,j,x,y,rho,rhou,rhov,E"]
+    for j in range(2, 2 + cfg.ny):
+        y = cfg.y_min + (float(j - 2) + 0.5) * dy
+        for i in range(2, 2 + cfg.nx):
+            x = cfg.x_min + (float(i - 2) + 0.5) * dx
+            rows.append(",".join([str(i), str(j), g(x), g(y)] + [g(v) for v in U[j, i]]))
+    return rows
+
+
+@pytest.mark.parametrize("precision, nthreads", [(6, 3), (17, 0), (3, 1)])
+def test_native_csv_writer_matches_per_cell_formatting(tmp_path, precision, nthreads):
+    """The library's multi-threaded writer gives the same text as formatting
+    every value with %.{p}g in cell order, on a grid whose rows span several
+    formatting blocks and rounds, with values of every sign and magnitude
+    (including -0, denormals and 1e300)."""
+    import numpy as np
+
+    from large_sys_eq_opt_amd.solver import write_csv_aos
+
+    cfg = cases.CaseConfig(300, 700, 2, -1.5, 2.0, 0.25, 3.0, 0.8, 0.0)
+    rng = np.random.default_rng(precision)
+    U = rng.standard_normal((cfg.ny + 4, cfg.nx + 4, 4)) * 10.0 ** rng.integers(-300, 300, (cfg.ny + 4, cfg.nx + 4, 4))
+    U[5, 5] = [-0.0, 0.0, 5e-324, -1e300]
+    U[7, 9] = [1.0, -1.0, 123456789.0, 0.1]
+    fn = str(tmp_path / "n.csv")
+    write_csv_aos(U, cfg, fn, precision, nthreads)
+    assert open(fn).read().splitlines() == _csv_rows_python(U, cfg, precision)
 
 
 def test_ensemble_spec_is_the_committed_table():

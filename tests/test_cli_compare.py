@@ -22,11 +22,13 @@ def test_run_options_defaults_and_flags():
     assert not o.write_output and o.num_snapshots == 0
     o = parse_run_options(["1", "--no-output", "--device-loop", "--precision", "17", "--outdir", "x"])
     assert o.device_loop and o.precision == 17 and o.outdir == "x"
+    o = parse_run_options(["1", "--device-loop", "--output"])
+    assert o.device_loop and o.write_output and o.num_snapshots == 5
 
 
 @pytest.mark.parametrize("argv", [["0"], ["-3"], ["two"], ["1", "--case"], ["1", "--case", "nope"],
                                   ["1", "--solver", "roe"], ["1", "--solver"], ["1", "--fast"],
-                                  ["1", "--device-loop", "--output"], ["1", "--precision", "30"]])
+                                  ["1", "--precision", "30"]])
 def test_run_options_errors(argv):
     from large_sys_eq_opt_amd.cli import main, parse_run_options
 

@@ -14,6 +14,10 @@ the device and the checks run there (torch on the copied interior):
   * the host-synchronous loop (dt from mh_compute_dt each step) and the
     device-resident loop (dt from the speed maximum fused into the y sweep)
     give the same dt sequence and the same state bit for bit.
+
+The benchmark's full horizon -- 200 steps of HLLC and of the exact solver at
+4096^2, 8192^2 and 16384^2 -- runs through all of these properties at once
+(test_full_size_200_steps).
 """
 from __future__ import annotations
 
@@ -22,6 +26,7 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 HLLC = 1
+EXACT = 2
 
 
 def _torch():
@@ -104,6 +109,51 @@ def test_full_size_blast_symmetry_and_conservation(hip, n, steps):
     assert abs(V[0].sum().item() - mass0) <= 1e-12 * mass0
     assert abs(V[3].sum().item() - energy0) <= 1e-12 * energy0
     g.close()
+
+
+@pytest.mark.parametrize("n, solver", [(4096, HLLC), (8192, HLLC), (16384, HLLC), (4096, EXACT), (8192, EXACT),
+                                       (16384, EXACT)])
+def test_full_size_200_steps(hip, n, solver):
+    """200 steps of the benchmark workload (the cylindrical blast on n x n
+    cells) in the device-resident loop: finite, mirror symmetric about both
+    centre lines, mass and energy conserved (the blast travels at most about
+    160 cells in 200 steps, far from the edges), and bit for bit the state
+    and dt sequence of the host-synchronous loop over the same 200 steps."""
+    from large_sys_eq_opt_amd import advance_second_order_gpu, compute_dt_gpu
+
+    torch = _torch()
+    steps = 200
+    U0 = _blast_state(torch, n)
+    mass0, energy0 = U0[0].sum().item(), U0[3].sum().item()
+    a = _grid_from(hip, torch, U0)
+    st = hip.Stepper(a, solver, 0.8, 0.0, None, log_capacity=steps)
+    st.advance(steps)
+    st.finish()
+    _, nsteps, dts_dev = st.clock()
+    st.close()
+    assert nsteps == steps and (dts_dev > 0).all()
+    V = _interior(hip, torch, a)
+    a.close()
+    assert torch.isfinite(V).all().item()
+    _check_symmetric(torch, V)
+    assert abs(V[0].sum().item() - mass0) <= 1e-12 * mass0
+    assert abs(V[3].sum().item() - energy0) <= 1e-12 * energy0
+    b = _grid_from(hip, torch, U0)
+    del U0
+    tmp, nxt = (hip.Grid2DGPU(n, n, 2, 0.0, 2.0, 0.0, 2.0) for _ in range(2))
+    dts_host = []
+    for _ in range(steps):
+        dt = compute_dt_gpu(b, None, 0.8)
+        dts_host.append(dt)
+        advance_second_order_gpu(b, tmp, nxt, None, dt, solver)
+        b.swap(nxt)
+    for g in (tmp, nxt):
+        g.close()
+    Vb = _interior(hip, torch, b)
+    b.close()
+    assert list(dts_dev) == dts_host
+    assert torch.equal(V, Vb)
+    print(f"{n}^2 solver {solver}: 200 steps, t = {sum(dts_host):.6g}", flush=True)
 
 
 def test_full_size_uniform_state_stays_uniform(hip):
@@ -189,7 +239,8 @@ def test_tensor_copies_check_dtype_and_shape(hip):
     g.close()
 
 
-@pytest.mark.parametrize("n, solver, steps", [(8192, 1, 3), (8192, 2, 2), (16384, 1, 1)])
+@pytest.mark.parametrize("n, solver, steps", [(4096, 1, 3), (4096, 2, 3), (8192, 1, 3), (8192, 2, 2), (16384, 1, 1),
+                                              (16384, 2, 1)])
 def test_full_size_steps_match_oracle(hip, oracle, n, solver, steps):
     """The benchmark's own workload at full size -- the cylindrical blast on
     n x n cells through the device-resident loop -- for a few steps against

@@ -332,23 +332,90 @@ def test_first_order_run(hip, oracle, solver):
         _assert_runs_match(o, g, solver)
 
 
-# The reference's own runs at scale 1 and their recorded total step counts
-# (validation/cpu_mpi_fullnode_external_timing_11488.csv:2-16).
-REFERENCE_STEP_COUNTS = {("shock_bubble", 0): 6359, ("shock_bubble", 1): 6913, ("shock_bubble", 2): 6912,
-                         ("shock_bubble", 3): 6539, ("blast_wave", 0): 2306, ("blast_wave", 1): 2335,
-                         ("blast_wave", 2): 2334, ("blast_wave", 3): 2308}
+# The reference's own runs and their recorded total step counts, for both
+# cases and all four solvers at scales 1, 2 and 4
+# (validation/cpu_mpi_fullnode_external_timing_11488.csv:2-48, the same counts
+# for the OpenMP and MPI runs), and its scale-8 HLL shock-bubble run on one
+# GPU (validation/gpu_clean_external_timing_11519.csv:5, 4000x1576).
+_SOLVER_NAMES = ("hll", "hllc", "exact", "force")
+_RECORDED = {
+    ("shock_bubble", 1): (6359, 6913, 6912, 6539), ("blast_wave", 1): (2306, 2335, 2334, 2308),
+    ("shock_bubble", 2): (13440, 14245, 14242, 13677), ("blast_wave", 2): (4710, 4765, 4764, 4713),
+    ("shock_bubble", 4): (27955, 28952, 28942, 28375), ("blast_wave", 4): (9617, 9742, 9736, 9616),
+}
+REFERENCE_STEP_COUNTS = {(case, s, n): steps[s] for (case, n), steps in _RECORDED.items() for s in range(4)}
+REFERENCE_STEP_COUNTS[("shock_bubble", 0, 8)] = 57314
 
 
-@pytest.mark.parametrize("case, solver", sorted(REFERENCE_STEP_COUNTS))
-def test_reference_step_counts_on_gpu(hip, case, solver):
+@pytest.mark.parametrize("case, solver, scale", sorted(REFERENCE_STEP_COUNTS, key=lambda k: (k[2], k[0], k[1])))
+def test_reference_step_counts_on_gpu(hip, case, solver, scale):
     """The reference's two cases -- the air/helium shock-bubble in SI units
-    (500x197) and the blast wave (500x500) -- run to their t_end through the
-    device-resident loop take exactly the step counts the reference recorded
-    for each solver (one rounding of any dt can change the count)."""
+    (500x197 times the scale) and the blast wave (500x500 times the scale) --
+    run to their t_end through the device-resident loop take exactly the step
+    counts the reference recorded for each solver and scale (one rounding of
+    any dt can change the count).  Printed for the record."""
     from large_sys_eq_opt_amd.driver import run_case
 
-    g = run_case(case, solver, 1, mode="device")
-    assert g["steps"] == REFERENCE_STEP_COUNTS[(case, solver)]
+    g = run_case(case, solver, scale, mode="device")
+    print(f"{case} {_SOLVER_NAMES[solver]} n={scale}: {g['steps']} steps "
+          f"(recorded {REFERENCE_STEP_COUNTS[(case, solver, scale)]}), {g['seconds']:.2f} s", flush=True)
+    assert g["steps"] == REFERENCE_STEP_COUNTS[(case, solver, scale)]
+
+
+def test_benchmark_matrix_runner_writes_reference_columns(hip, tmp_path):
+    """The matrix runner (scripts/slurm/gpu_all.sh:183-234) at scale 1: one
+    row per case and solver in the reference's summary columns, each with
+    the recorded step count."""
+    from large_sys_eq_opt_amd.matrix import HEADER, run_matrix
+
+    csv = tmp_path / "gpu_matrix.csv"
+    rows = run_matrix([1], ["shock_bubble", "blast_wave"], list(_SOLVER_NAMES), str(csv), str(tmp_path / "logs"),
+                      "t", log=None)
+    lines = csv.read_text().splitlines()
+    assert lines[0] == HEADER == ("arch,case,solver,n,nx,ny,total_cells,total_steps,real_seconds,user_seconds,"
+                                  "sys_seconds,max_rss_kb,exit_status,log_file")
+    assert len(rows) == len(lines) - 1 == 8
+    for r in rows:
+        s = _SOLVER_NAMES.index(r["solver"])
+        assert r["exit_status"] == "0"
+        assert int(r["total_steps"]) == REFERENCE_STEP_COUNTS[(r["case"], s, 1)]
+        assert int(r["total_cells"]) == int(r["nx"]) * int(r["ny"])
+        assert (r["nx"], r["ny"]) == (("500", "197") if r["case"] == "shock_bubble" else ("500", "500"))
+
+
+@pytest.mark.parametrize("solver", SOLVERS)
+def test_device_loop_snapshots_match_host_loop(hip, solver):
+    """Snapshots from the device-resident loop (the clock holds at each
+    snapshot time; the state goes to pinned host memory on a copy stream
+    while the next steps run) equal the reference-compatible host loop's bit
+    for bit -- the full 1024x512 shock-bubble case to t = 0.4 with 5
+    snapshots: snapshot times, every snapshot (ghosts included), dt sequence
+    and final state."""
+    from large_sys_eq_opt_amd.driver import run_case
+
+    h = run_case("shock_bubble_nd", solver, 1, mode="host", num_snapshots=5, keep_snapshots=True)
+    d = run_case("shock_bubble_nd", solver, 1, mode="device", num_snapshots=5, keep_snapshots=True)
+    assert d["steps"] == h["steps"]
+    assert same_bits(d["dt"], h["dt"])
+    assert same_bits(d["snap_t"], h["snap_t"]) and len(d["snap_t"]) == 5
+    assert len(d["snapshots"]) == len(h["snapshots"]) == 5
+    for k in range(5):
+        assert same_bits(d["snapshots"][k], h["snapshots"][k]), f"snapshot {k + 1}"
+    assert same_bits(d["U"], h["U"])
+
+
+def test_device_loop_max_steps_past_t_end(hip):
+    """max_steps beyond the case's t_end: the device loop stops and clamps at
+    t_end exactly as the host loop does (same step count, dt sequence and
+    state)."""
+    from large_sys_eq_opt_amd.driver import run_case
+
+    h = run_case("cyl_blast", 1, 1, mode="host", max_steps=100000)
+    d = run_case("cyl_blast", 1, 1, mode="device", max_steps=100000)
+    assert h["t"] == d["t"] == 0.25
+    assert d["steps"] == h["steps"] < 100000
+    assert same_bits(d["dt"], h["dt"])
+    assert same_bits(interior(d["U"]), interior(h["U"]))
 
 
 @pytest.mark.parametrize("case", ["shock_bubble", "blast_wave"])
@@ -363,17 +430,19 @@ def test_reference_cases_match_oracle(hip, oracle, case):
     U0 = oracle.init(kind, params.array(), cfg.nx, cfg.ny, cfg.bounds)
     o = oracle.run(U0, cfg.dx, cfg.dy, 1, cfg.cfl, t_end=cfg.t_end)
     g = run_case(case, 1, 1, mode="device")
-    assert g["steps"] == o["steps"] == REFERENCE_STEP_COUNTS[(case, 1)]
+    assert g["steps"] == o["steps"] == REFERENCE_STEP_COUNTS[(case, 1, 1)]
     _assert_runs_match(o, g, 1)
 
 
-def test_reference_snapshot_times_on_gpu(hip):
+@pytest.mark.parametrize("mode", ["host", "device"])
+def test_reference_snapshot_times_on_gpu(hip, mode):
     """The reference's printed snapshot times of the shock-bubble HLL run
     (outputs/baseline/cpu_stdout.txt:1-5, 6 significant digits), from the
-    host-synchronous loop with 5 snapshots (first step with t >= k t_end / 5)."""
+    host-synchronous loop and from the device loop's snapshot stops, with 5
+    snapshots (first step with t >= k t_end / 5)."""
     from large_sys_eq_opt_amd.driver import run_case
 
-    g = run_case("shock_bubble", 0, 1, num_snapshots=5)
-    assert g["steps"] == REFERENCE_STEP_COUNTS[("shock_bubble", 0)]
+    g = run_case("shock_bubble", 0, 1, mode=mode, num_snapshots=5)
+    assert g["steps"] == REFERENCE_STEP_COUNTS[("shock_bubble", 0, 1)]
     rec = [0.000234916, 0.000469817, 0.000704535, 0.000939401, 0.0011741]
     assert [float(f"{t:.6g}") for t in g["snap_t"]] == rec
