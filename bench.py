@@ -15,20 +15,26 @@ roofline: the dominant kernel (the slower sweep) -- 64 algorithmic bytes per
   (128 B per cell update) is reported beside it.
 cpu_baseline: the reference CPU solver compiled from its sources
   (oracle/_ref) -- or the oracle port when that build is absent -- on the
-  same workload at 2048^2 for a bounded number of steps, with all host
-  threads OpenMP gives it.
+  same workload (8192^2 HLLC) for a bounded sample (median of 3 runs of a
+  few steps, ~20 s in all), OpenMP threads bound to cores; the CPU model and
+  core counts are recorded.
 batched: 512 blast-wave instances (seed 0, sides log-uniform in [64,1024],
-  95.3 M cells), 500 steps with per-instance dt, one persistent launch per
-  step replayed from captured hipGraphs, schedule from the autotuner; its
-  cell-update rate and the ratio to the single-grid value.
+  95.3 M cells), 500 steps with per-instance dt replayed from captured
+  hipGraphs under the autotuner's schedule (the line names it: "mode" is
+  dispatched launches or the persistent kernel), its cell-update rate and
+  ratio to the single-grid value; the persistent kernel (one launch per step,
+  grid barrier between the phases) is timed over the same 500 steps beside
+  it ("persistent").
 adjoint: the 512x256 shock-bubble adjoint case (400 HLLC steps, a
   checkpoint every 20): the backward pass (segment recompute + adjoint
-  steps) per step over the device-resident forward step.
+  steps) per step over the device-resident forward step, and the whole
+  gradient (checkpointing forward + backward) over it.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--size N]
-       [--solver hllc] [--no-cpu] [--no-extras] [--leg batched|adjoint]
-(--leg runs one of the two extra legs alone and prints its JSON object:
-the profiling runs of those workloads use it.)
+       [--solver hllc] [--no-cpu] [--no-extras] [--leg batched|adjoint|snapshot]
+(--leg runs one extra leg alone and prints its JSON object: the profiling
+runs of those workloads use it; "snapshot" measures the cost of the device
+loop's snapshots at --size.)
 """
 from __future__ import annotations
 
@@ -47,46 +53,106 @@ BYTES_PER_SWEEP = 64.0   # read + write of four FP64 planes per cell
 METRIC = "Gcell-updates/s + % HBM roofline at 8192² FP64; batched/adjoint vs plain"
 
 
-def cpu_baseline(solver: int, budget_s: float = 12.0):
-    """Reference CPU solver (or the oracle port) on cyl_blast at 2048^2."""
+def host_cpu() -> dict:
+    """The host's CPU model and core counts (lscpu), and how many of its
+    logical CPUs this process may run on (its affinity mask)."""
+    info = {"model": None, "physical_cores": None, "logical_cpus": os.cpu_count(),
+            "affinity_cpus": len(os.sched_getaffinity(0))}
+    try:
+        txt = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        for line in txt.splitlines():
+            if line.startswith("Model name:"):
+                info["model"] = line.split(":", 1)[1].strip()
+        par = subprocess.run(["lscpu", "-p=CPU,CORE,SOCKET"], capture_output=True, text=True, timeout=20).stdout
+        cores, mine = set(), set()
+        allowed = os.sched_getaffinity(0)
+        for line in par.splitlines():
+            if line.startswith("#"):
+                continue
+            cpu, core, sock = (int(x) for x in line.split(",")[:3])
+            cores.add((sock, core))
+            if cpu in allowed:
+                mine.add((sock, core))
+        info["physical_cores"] = len(cores)
+        info["physical_cores_in_affinity"] = len(mine)
+    except (OSError, ValueError, subprocess.SubprocessError):
+        pass
+    return info
+
+
+def cpu_baseline(solver: int, n: int = 8192, samples: int = 3, budget_s: float = 20.0):
+    """The reference CPU solver compiled from its sources (oracle/_ref; the
+    oracle port where that build is absent) on the benchmark workload itself
+    -- cyl_blast at n^2 -- for a bounded sample: one calibration step, then
+    `samples` runs of k steps sized to ~budget_s in total; the median rate.
+    OpenMP threads: OMP_NUM_THREADS as the environment sets it (else the
+    physical cores this process may use), bound close to cores
+    (OMP_PROC_BIND=close, OMP_PLACES=cores, set before the library loads, as
+    scripts/slurm/cpu_all.sh:110-111 does)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import numpy as np
     import pyoracle
 
     from large_sys_eq_opt_amd import cases
 
-    cfg = cases.get_n_case_config("cyl_blast", 8)   # 2048^2
+    cfg = cases.get_n_case_config("cyl_blast", n // 256)
     kind, params = cases.case_params("cyl_blast")
     orc = pyoracle.Oracle()
     U0 = orc.init(kind, params.array(), cfg.nx, cfg.ny, cfg.bounds)
     if pyoracle.reference_available():
         impl, kind_s = pyoracle.Reference(), "reference"
-        run = lambda n: impl.run(U0, cfg.bounds, solver, cfg.cfl, max_steps=n)  # noqa: E731
+        run = lambda k: impl.run(U0, cfg.bounds, solver, cfg.cfl, max_steps=k)  # noqa: E731
     else:
         impl, kind_s = orc, "port"
-        run = lambda n: impl.run(U0, cfg.dx, cfg.dy, solver, cfg.cfl, max_steps=n)  # noqa: E731
-    probe = run(2)
-    per_step = max(probe["seconds"] / 2.0, 1e-6)
-    n = int(max(3, min(400, budget_s / per_step)))
-    r = run(n)
-    cells = cfg.nx * cfg.ny
-    rate = cells * r["steps"] / r["seconds"]
+        run = lambda k: impl.run(U0, cfg.dx, cfg.dy, solver, cfg.cfl, max_steps=k)  # noqa: E731
+    probe = run(1)
+    k = int(max(1, min(200, budget_s / samples / max(probe["seconds"], 1e-6))))
+    rates = []
+    for _ in range(samples):
+        r = run(k)
+        rates.append(cfg.nx * cfg.ny * r["steps"] / r["seconds"])
+    rate = float(np.median(rates))
     threads = orc.num_threads()
+    cpu = _HOST_CPU or host_cpu()
+    omp = {v: os.environ.get(v) for v in ("OMP_NUM_THREADS", "OMP_PROC_BIND", "OMP_PLACES")}
     return {"value": rate / 1e9, "unit": "Gcell-updates/s", "cores": threads, "kind": kind_s,
-            "sample": f"cyl_blast 2048x2048 HLLC FP64, {r['steps']} steps, {r['seconds']:.2f} s, "
-                      f"{threads} OpenMP threads ({'reference CPU sources' if kind_s == 'reference' else 'oracle port'})",
-            "np": np.__version__}
+            "sample": f"cyl_blast {cfg.nx}x{cfg.ny} {['HLL', 'HLLC', 'exact', 'FORCE'][solver]} FP64 (the benchmark "
+                      f"workload), median of {samples} runs of {k} steps from the initial state "
+                      f"({', '.join(f'{x / 1e9:.4f}' for x in rates)} Gcell-updates/s), {threads} OpenMP threads, "
+                      f"{'reference CPU sources' if kind_s == 'reference' else 'oracle port'}",
+            "cpu_model": cpu["model"], "physical_cores": cpu.get("physical_cores"),
+            "physical_cores_in_affinity": cpu.get("physical_cores_in_affinity"),
+            "affinity_cpus": cpu["affinity_cpus"], "omp_env": omp}
 
 
-def batched_leg(plain_rate: float, steps: int = 500):
-    """512-instance ensemble, graph-replayed persistent launches."""
+_HOST_CPU = None
+
+
+def _set_omp_env():
+    """OpenMP placement of the CPU baseline, set before any OpenMP runtime
+    loads (and the host's CPU read before any thread is bound).  An
+    OMP_NUM_THREADS already in the environment is kept."""
+    global _HOST_CPU
+    _HOST_CPU = host_cpu()
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    os.environ.setdefault("OMP_PLACES", "cores")
+    if "OMP_NUM_THREADS" not in os.environ:
+        n = _HOST_CPU.get("physical_cores_in_affinity") or _HOST_CPU["affinity_cpus"]
+        os.environ["OMP_NUM_THREADS"] = str(max(1, n))
+
+
+MODES = {0: "persistent: one launch per step, grid barrier between the x and y phases",
+         1: "persistent: two launches per step (x phase, y phase)",
+         2: "dispatched: two launches per step (x tiles; y tiles + IEEE redo), one tile per wavefront"}
+
+
+def _timed_batch(spec, schedule, steps, per_graph=25, size_class=(0, 0, 0)):
     from large_sys_eq_opt_amd import ensemble
 
-    spec = ensemble.ensemble_spec(512, 0)
     b = ensemble.Batch(spec, solver=1, cfl=0.8)
+    b.set_schedule(*schedule)
+    b.set_size_class(*size_class)
     b.initialise()
-    tune = ensemble.autotune(b)
-    per_graph = 25
     b.capture(per_graph)
     b.clock()
     t0 = time.perf_counter()
@@ -94,12 +160,34 @@ def batched_leg(plain_rate: float, steps: int = 500):
     _, _, st, _ = b.clock()
     wall = time.perf_counter() - t0
     info = b.info()
+    cells = b.cells
     b.close()
-    rate = b.cells * steps / wall
-    return {"value": rate / 1e9, "unit": "Gcell-updates/s", "instances": len(spec), "cells": b.cells,
-            "steps": steps, "ms_per_step": wall * 1e3 / steps, "ratio_vs_plain": rate / plain_rate,
-            "schedule": list(tune["schedule"]), "graph_steps": per_graph, "blocks": info["blocks"],
-            "all_instances_stepped": bool((st == steps).all())}
+    return cells * steps / wall, wall, info, bool((st == steps).all())
+
+
+def batched_leg(plain_rate: float, steps: int = 500):
+    """512-instance ensemble, graph-replayed, under the autotuned schedule and
+    under the persistent kernel."""
+    from large_sys_eq_opt_amd import ensemble
+
+    spec = ensemble.ensemble_spec(512, 0)
+    b = ensemble.Batch(spec, solver=1, cfl=0.8)
+    b.initialise()
+    tune = ensemble.autotune(b)
+    b.close()
+    sched, cls = tuple(tune["schedule"]), tuple(tune["size_class"])
+    per_graph = 25
+    rate, wall, info, ok = _timed_batch(spec, sched, steps, per_graph, cls)
+    pers = min((t, c) for c, t in tune["timings"] if c[2] == 0)[1]
+    prate, pwall, pinfo, pok = _timed_batch(spec, pers, steps, per_graph)
+    return {"value": rate / 1e9, "unit": "Gcell-updates/s", "instances": len(spec),
+            "cells": sum(x["side"] ** 2 for x in spec), "steps": steps, "ms_per_step": wall * 1e3 / steps,
+            "ratio_vs_plain": rate / plain_rate, "schedule": list(sched), "mode": MODES[sched[2]],
+            "size_class": {"side_below": cls[0], "ychunk": cls[1], "xrows": cls[2]},
+            "graph_steps": per_graph, "blocks": info["blocks"], "all_instances_stepped": ok,
+            "persistent": {"schedule": list(pers), "mode": MODES[0], "value": prate / 1e9,
+                           "ms_per_step": pwall * 1e3 / steps, "ratio_vs_plain": prate / plain_rate,
+                           "blocks": pinfo["blocks"], "all_instances_stepped": pok}}
 
 
 def adjoint_leg():
@@ -139,7 +227,83 @@ def adjoint_leg():
     return {"workload": f"shock_bubble_adj {cfg.nx}x{cfg.ny} HLLC, {N} steps, checkpoint every 20",
             "forward_us_per_step": f * 1e6, "checkpointing_forward_us_per_step": c * 1e6,
             "backward_us_per_step": b * 1e6, "ratio_vs_forward": b / f,
-            "ratio_vs_checkpointing_forward": b / c}
+            "ratio_vs_checkpointing_forward": b / c, "whole_gradient_ratio_vs_forward": (c + b) / f}
+
+
+def snapshot_leg(n: int = 8192, solver: int = 1, csv_dir: str | None = None):
+    """Snapshot cost on the benchmark workload (cyl_blast n^2, run to
+    t_end = 0.25 in the device-resident loop): the loop alone; the same loop
+    holding at the 5 snapshot times, each snapshot copied on the device and
+    then to pinned host memory on the copy stream while the next steps run;
+    and one snapshot written as the reference's CSV by the library's
+    writer (to csv_dir or a temporary directory, removed afterwards; skipped
+    without 4x its size free)."""
+    import shutil
+    import tempfile
+
+    from large_sys_eq_opt_amd import Grid2DGPU, Snapshot, Stepper, cases
+
+    cfg = cases.get_n_case_config("cyl_blast", n // 256)
+    kind, params = cases.case_params("cyl_blast")
+    bc = cases.case_boundary("cyl_blast")
+    times = [cfg.t_end * k / 5.0 for k in range(1, 6)]
+    out = {"workload": f"cyl_blast {n}x{n} HLLC to t_end {cfg.t_end}, device-resident loop, 5 snapshots"}
+    U = Grid2DGPU.from_config(cfg)
+    snap = None
+    for with_snaps in (False, True):
+        U.init_state(kind, params.array())
+        st = Stepper(U, solver, cfg.cfl, cfg.t_end, bc)
+        if with_snaps and snap is None:
+            snap = Snapshot(U, 2)
+        t0 = time.perf_counter()
+        nk, t, steps, waits = 0, 0.0, 0, 0.0
+        while True:
+            if with_snaps:
+                st.set_stop(times[nk] if nk < 5 else 0.0)
+            st.advance(256)
+            t, steps, _ = st.clock()
+            while with_snaps and nk < 5 and t >= times[nk]:
+                if nk >= 2:   # a slot is reused: its previous copy must have landed
+                    w0 = time.perf_counter()
+                    snap.planes(nk % 2)
+                    waits += time.perf_counter() - w0
+                st.finish()
+                snap.capture(U, nk % 2)
+                nk += 1
+            if t >= cfg.t_end:
+                break
+        if with_snaps:
+            snap.planes(0)
+            snap.planes(1)
+        wall = time.perf_counter() - t0
+        st.close()
+        key = "loop_with_5_snapshots" if with_snaps else "loop"
+        out[key] = {"seconds": wall, "steps": steps}
+        if with_snaps:
+            out[key]["host_wait_seconds"] = waits
+    out["snapshot_overhead_seconds"] = out["loop_with_5_snapshots"]["seconds"] - out["loop"]["seconds"]
+    out["state_bytes"] = 4 * 8 * (cfg.nx + 4) * (cfg.ny + 4)
+    d = csv_dir or tempfile.mkdtemp(prefix="mh_snap_")
+    try:
+        need = 4 * 70 * cfg.nx * cfg.ny
+        free = shutil.disk_usage(d).free
+        if free < need:
+            out["csv"] = {"skipped": f"{free / 1e9:.1f} GB free in {d}, about {need / 4e9:.1f} GB needed"}
+        else:
+            fn = os.path.join(d, "gpu_cyl_blast_hllc_snapshot_5.csv")
+            t0 = time.perf_counter()
+            snap.write_csv(1, fn, 6)
+            w = time.perf_counter() - t0
+            size = os.path.getsize(fn)
+            out["csv"] = {"seconds": w, "bytes": size, "MB_per_s": size / w / 1e6, "rows": cfg.nx * cfg.ny,
+                          "threads": min(16, os.cpu_count() or 1)}
+            os.remove(fn)
+    finally:
+        if not csv_dir:
+            shutil.rmtree(d, ignore_errors=True)
+    snap.close()
+    U.close()
+    return out
 
 
 def copy_peak(nbytes: int = 4 << 30, reps: int = 10):
@@ -165,6 +329,7 @@ def copy_peak(nbytes: int = 4 << 30, reps: int = 10):
 
 
 def main():
+    _set_omp_env()
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
@@ -173,7 +338,7 @@ def main():
     ap.add_argument("--solver", default="hllc")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="skip the batched and adjoint legs")
-    ap.add_argument("--leg", choices=["batched", "adjoint"], default=None,
+    ap.add_argument("--leg", choices=["batched", "adjoint", "snapshot"], default=None,
                     help="run only this extra leg (batched: against --plain-rate Gcell/s)")
     ap.add_argument("--plain-rate", type=float, default=32.0)
     ap.add_argument("--copy-only", type=int, default=0,
@@ -192,6 +357,9 @@ def main():
         return
     if args.leg == "adjoint":
         print(json.dumps(adjoint_leg()))
+        return
+    if args.leg == "snapshot":
+        print(json.dumps(snapshot_leg(args.size)))
         return
 
     from large_sys_eq_opt_amd import Grid2DGPU, Stepper, cases, parse_riemann_solver
@@ -261,8 +429,7 @@ def main():
         out["batched"] = batched_leg(rate)
         out["adjoint"] = adjoint_leg()
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(int(solver))
-        out["cpu_baseline"].pop("np", None)
+        out["cpu_baseline"] = cpu_baseline(int(solver), n)
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out))

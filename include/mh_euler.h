@@ -231,6 +231,11 @@ int mh_batch_prepare(mh_batch b, mh_stream stream);
  * launches with one tile per wavefront), persistent workgroups per CU (0 =
  * occupancy limit).  Drops a captured graph. */
 int mh_batch_set_schedule(mh_batch b, int xseg, int xrows, int ychunk, int mode, int blocks_per_cu);
+/* Size class of the schedule: instances with fewer than side_below cells a
+ * side (nx) take y chunks of ychunk rows and x tiles of xrows rows instead
+ * (0 keeps the schedule's value; side_below 0 turns the class off).  Drops
+ * a captured graph. */
+int mh_batch_set_size_class(mh_batch b, int side_below, int ychunk, int xrows);
 /* Enqueues n steps of every instance (graph replays where captured). */
 int mh_batch_step(mh_batch b, int nsteps, mh_stream stream);
 /* Captures steps_per_graph steps into a hipGraph used by mh_batch_step. */

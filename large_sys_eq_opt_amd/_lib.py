@@ -73,6 +73,7 @@ SIGNATURES = {
     "mh_batch_grid": ([_P, _I, C.POINTER(_P)], _I),
     "mh_batch_prepare": ([_P, _P], _I),
     "mh_batch_set_schedule": ([_P, _I, _I, _I, _I, _I], _I),
+    "mh_batch_set_size_class": ([_P, _I, _I, _I], _I),
     "mh_batch_step": ([_P, _I, _P], _I),
     "mh_batch_capture": ([_P, _I, _P], _I),
     "mh_batch_clock": ([_P, _P, _P, _P, _P, _P], _I),

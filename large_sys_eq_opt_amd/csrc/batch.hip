@@ -160,7 +160,8 @@ __global__ __launch_bounds__(256) void k_batch_step(BatchArgs a) {
             double dt;
             if (!inst_dt(I, a.clk[t.inst], par, dt)) continue;
             const double dtd = dt / I.dx;
-            for (int j = t.j0; j < t.j1; ++j) x_tile<S, 2>(I.U, I.T, j, t.i0, t.i1, dtd, 0.5 * dtd, I.bc, 1, X[w].x, lane);
+            for (int j = t.j0; j < t.j1; ++j)
+                x_tile<S, 2, false>(I.U, I.T, j, t.i0, t.i1, dtd, 0.5 * dtd, I.bc, 1, X[w].x, lane);
         }
     }
     // phase y rewrites U: never on top of an incomplete x phase
@@ -172,7 +173,7 @@ __global__ __launch_bounds__(256) void k_batch_step(BatchArgs a) {
             double dt;
             if (!inst_dt(I, a.clk[t.inst], par, dt)) continue;
             const double dtd = dt / I.dy;
-            double m = y_tile<S>(I.T, I.U, t.cg, t.j0, t.j1, dtd, 0.5 * dtd, I.bc, 1, 1, X[w].ring, lane);
+            double m = y_tile<S, false>(I.T, I.U, t.cg, t.j0, t.j1, dtd, 0.5 * dtd, I.bc, 1, 1, X[w].ring, lane);
             m = wave_max(m);
             if (lane == 0 && m > 0.0)
                 atomicMax(&a.clk[t.inst].smax[par ^ 1], (unsigned long long)__double_as_longlong(m));
@@ -203,7 +204,7 @@ __global__ __launch_bounds__(64) void k_batch_x(BatchArgs a, int ntiles) {
     double dt;
     if (!inst_dt(I, a.clk[t.inst], par, dt)) return;
     const double dtd = dt / I.dx;
-    for (int j = t.j0; j < t.j1; ++j) x_tile<S, 2>(I.U, I.T, j, t.i0, t.i1, dtd, 0.5 * dtd, I.bc, 1, X[w], lane);
+    for (int j = t.j0; j < t.j1; ++j) x_tile<S, 2, false>(I.U, I.T, j, t.i0, t.i1, dtd, 0.5 * dtd, I.bc, 1, X[w], lane);
 }
 
 // Dispatched y phase: wavefront q runs y tile q with the fast division
@@ -227,7 +228,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 8))) voi
     const double dtd = dt / I.dy;
     double m = 0.0;
     if (dt_in_fast_range(dtd) &&
-        y_tile_pass<S, true>(I.T, I.U, t.cg, t.j0, t.j1, dtd, 0.5 * dtd, I.bc, 1, 1, ring[w], lane, m)) {
+        y_tile_pass<S, true, false>(I.T, I.U, t.cg, t.j0, t.j1, dtd, 0.5 * dtd, I.bc, 1, 1, ring[w], lane, m)) {
         m = wave_max(m);
         if (lane == 0 && m > 0.0) atomicMax(&a.clk[t.inst].smax[par ^ 1], (unsigned long long)__double_as_longlong(m));
     } else if (lane == 0) {
@@ -251,7 +252,7 @@ __global__ __launch_bounds__(256) void k_batch_y_redo(BatchArgs a) {
         if (!inst_dt(I, a.clk[t.inst], par, dt)) continue;
         const double dtd = dt / I.dy;
         double m = 0.0;
-        y_tile_pass<S, false>(I.T, I.U, t.cg, t.j0, t.j1, dtd, 0.5 * dtd, I.bc, 1, 1, ring[w], lane, m);
+        y_tile_pass<S, false, false>(I.T, I.U, t.cg, t.j0, t.j1, dtd, 0.5 * dtd, I.bc, 1, 1, ring[w], lane, m);
         m = wave_max(m);
         if (lane == 0 && m > 0.0) atomicMax(&a.clk[t.inst].smax[par ^ 1], (unsigned long long)__double_as_longlong(m));
     }
@@ -307,6 +308,9 @@ struct mh_batch_s {
     // schedule; mode 0: one persistent launch per step, 1: two persistent
     // launches, 2: two dispatched launches (one tile per wavefront)
     int xseg = 0, xrows = 1, ychunk = 256, mode = 0, blocks_per_cu = 0;
+    // size class: instances with fewer than small_side cells a side take
+    // their own y chunk rows and x tile rows (0: the schedule's)
+    int small_side = 0, small_ychunk = 0, small_xrows = 0;
     hipGraphExec_t graph = nullptr;
     hipGraph_t graph_def = nullptr;
     int graph_steps = 0;
@@ -382,15 +386,17 @@ int build_schedule(mh_batch b) {
     });
     for (int k : order) {
         const int nx = b->hinst[k].U.nx, ny = b->hinst[k].U.ny;
+        const bool small = b->small_side > 0 && nx < b->small_side;
         const int seg = b->xseg > 0 ? b->xseg : nx;
-        const int rows = std::max(1, b->xrows);
+        const int rows = std::max(1, small && b->small_xrows > 0 ? b->small_xrows : b->xrows);
+        const int ychunk = small && b->small_ychunk > 0 ? b->small_ychunk : b->ychunk;
         for (int j0 = 0; j0 < ny; j0 += rows)
             for (int i0 = 0; i0 < nx; i0 += seg) {
                 const int i1 = std::min(nx, i0 + seg), j1 = std::min(ny, j0 + rows);
                 xt.push_back({k, j0, j1, i0, i1});
                 xc.push_back(64.0 * (j1 - j0) * ((i1 - i0 + 4 + 63) / 64));
             }
-        const int ch = b->ychunk > 0 ? b->ychunk : ny;
+        const int ch = ychunk > 0 ? ychunk : ny;
         for (int cg = 0; cg * 64 < nx; ++cg)
             for (int j0 = 0; j0 < ny; j0 += ch) {
                 const int j1 = std::min(ny, j0 + ch);
@@ -590,6 +596,14 @@ int mh_batch_set_schedule(mh_batch b, int xseg, int xrows, int ychunk, int mode,
     b->ychunk = ychunk;
     b->mode = mode;
     b->blocks_per_cu = blocks_per_cu;
+    return build_schedule(b);
+}
+
+int mh_batch_set_size_class(mh_batch b, int side_below, int ychunk, int xrows) {
+    if (!b || side_below < 0 || ychunk < 0 || xrows < 0) return fail(MH_ERR_INVALID, "mh_batch_set_size_class: bad argument");
+    b->small_side = side_below;
+    b->small_ychunk = ychunk;
+    b->small_xrows = xrows;
     return build_schedule(b);
 }
 

@@ -53,16 +53,24 @@ __device__ __forceinline__ bool finite(double x) { return __builtin_isfinite(x);
 // p/(rho (S - u)) of the HLLC star state (a zero denominator there has
 // returned before).  Quotients whose numerator may be +-0 (HLL, S*, rho*)
 // keep the fixup, which gives the zero its IEEE sign.
-__device__ __forceinline__ double div_nr_nf(double a, double b) {
+// rcp_nr(b) is the refined reciprocal div_nr_nf forms before its quotient
+// steps, and div_rr(a, b, rcp_nr(b)) those steps: together exactly
+// div_nr_nf(a, b).  Quotients that share a denominator (the conversion's
+// 1/rho, the sound speed's gamma p/rho and HLLC's E/rho of one state) can
+// so share the five instructions of the reciprocal.
+__device__ __forceinline__ double rcp_nr(double b) {
     double r = __builtin_amdgcn_rcp(b);
     double e = __builtin_fma(-b, r, 1.0);
     r = __builtin_fma(r, e, r);
     e = __builtin_fma(-b, r, 1.0);
-    r = __builtin_fma(r, e, r);
+    return __builtin_fma(r, e, r);
+}
+__device__ __forceinline__ double div_rr(double a, double b, double r) {
     const double q = a * r;
     const double t = __builtin_fma(-b, q, a);
     return __builtin_fma(t, r, q);
 }
+__device__ __forceinline__ double div_nr_nf(double a, double b) { return div_rr(a, b, rcp_nr(b)); }
 __device__ __forceinline__ double div_nr(double a, double b) {
     return __builtin_amdgcn_div_fixup(div_nr_nf(a, b), b, a);
 }
@@ -130,6 +138,22 @@ __device__ __forceinline__ Prim c2p(const Cons& U) {
     const double kinetic = 0.5 * U.r * (u * u + v * v);
     return {U.r, u, v, kGm1 * (U.e - kinetic)};
 }
+// The same; with F, rr receives rcp_nr(U.r) (the conversion's 1/rho is
+// div_rr(1, rho, rr)).
+template <bool F = false>
+__device__ __forceinline__ Prim c2p_rr(const Cons& U, double& rr) {
+    if constexpr (F) {
+        rr = rcp_nr(U.r);
+        const double inv_rho = div_rr(1.0, U.r, rr);
+        const double u = U.mu * inv_rho;
+        const double v = U.mv * inv_rho;
+        const double kinetic = 0.5 * U.r * (u * u + v * v);
+        return {U.r, u, v, kGm1 * (U.e - kinetic)};
+    } else {
+        rr = 0.0;
+        return c2p<false>(U);
+    }
+}
 
 // head/physics.hpp:22-28
 template <bool F = false>
@@ -140,6 +164,12 @@ __device__ __forceinline__ Cons p2c(const Prim& V) {
 
 template <bool F = false>
 __device__ __forceinline__ double sound(const Prim& V) { return sq<F>(dvn<F>(kGamma * V.p, V.r)); }  // :39-41
+// sound speed of V with R: from rr = rcp_nr(V.r) (fast tiles only)
+template <bool F, bool R>
+__device__ __forceinline__ double sound_r(const Prim& V, double rr) {
+    if constexpr (F && R) return sqrt_nr(div_rr(kGamma * V.p, V.r, rr));
+    else return sound<F>(V);
+}
 
 // Physical flux of U along D given V = c2p(U) (head/physics.hpp:56-74).
 template <int D> __device__ __forceinline__ Cons pflux(const Cons& U, const Prim& V) {
@@ -172,12 +202,14 @@ __device__ __forceinline__ Cons avg(const Cons& a, const Cons& b) {  // 0.5 * (a
 // ---------------------------------------------------------------------------
 // HLL (head/riemann.hpp:87-129); VL/VR = c2p(UL/UR).
 // ---------------------------------------------------------------------------
-template <int D, bool F = false>
-__device__ __forceinline__ Cons hll(const Cons& UL, const Prim& VL, const Cons& UR, const Prim& VR) {
+// R (fast tiles): rrL, rrR are rcp_nr of the two densities.
+template <int D, bool F = false, bool R = false>
+__device__ __forceinline__ Cons hll(const Cons& UL, const Prim& VL, const Cons& UR, const Prim& VR, double rrL = 0.0,
+                                    double rrR = 0.0) {
     const Cons FL = pflux<D>(UL, VL);
     const Cons FR = pflux<D>(UR, VR);
     if (!phys_riemann<F>(VL) || !phys_riemann<F>(VR)) return avg(FL, FR);
-    const double aL = sound<F>(VL), aR = sound<F>(VR);
+    const double aL = sound_r<F, R>(VL, rrL), aR = sound_r<F, R>(VR, rrR);
     const double unL = un_of<D>(VL), unR = un_of<D>(VR);
     const double SL = lmin(unL - aL, unR - aR);
     const double SR = lmax(unL + aL, unR + aR);
@@ -195,25 +227,28 @@ __device__ __forceinline__ Cons hll(const Cons& UL, const Prim& VL, const Cons& 
 // ---------------------------------------------------------------------------
 // HLLC (head/riemann.hpp:135-271)
 // ---------------------------------------------------------------------------
-template <int D, bool F = false>
-__device__ __forceinline__ Cons hllc_star(const Cons& U, const Prim& V, double S, double Sstar) {
+template <int D, bool F = false, bool R = false>
+__device__ __forceinline__ Cons hllc_star(const Cons& U, const Prim& V, double S, double Sstar, double rr = 0.0) {
     const double rho = V.r, p = V.p;
     const double un = un_of<D>(V), ut = ut_of<D>(V);
     const double denom = S - Sstar;
     if (fabs(denom) < 1.0e-14) return U;
     const double rho_star = dvf<F>(rho * (S - un), denom);
     if ((!F && !finite(rho_star)) || rho_star <= 0.0) return U;   // finite in a fast tile
-    const double E_per_rho = dvn<F>(U.e, rho);
+    double E_per_rho;
+    if constexpr (F && R) E_per_rho = div_rr(U.e, rho, rr);
+    else E_per_rho = dvn<F>(U.e, rho);
     const double E_star = rho_star * (E_per_rho + (Sstar - un) * (Sstar + dvn<F>(p, rho * (S - un))));
     if (!finite(E_star)) return U;
     if (D == kX) return {rho_star, rho_star * Sstar, rho_star * ut, E_star};
     return {rho_star, rho_star * ut, rho_star * Sstar, E_star};
 }
 
-template <int D, bool F = false>
-__device__ __forceinline__ Cons hllc(const Cons& UL, const Prim& VL, const Cons& UR, const Prim& VR) {
-    if (!phys_riemann<F>(VL) || !phys_riemann<F>(VR)) return hll<D, F>(UL, VL, UR, VR);
-    const double aL = sound<F>(VL), aR = sound<F>(VR);
+template <int D, bool F = false, bool R = false>
+__device__ __forceinline__ Cons hllc(const Cons& UL, const Prim& VL, const Cons& UR, const Prim& VR, double rrL = 0.0,
+                                     double rrR = 0.0) {
+    if (!phys_riemann<F>(VL) || !phys_riemann<F>(VR)) return hll<D, F, R>(UL, VL, UR, VR, rrL, rrR);
+    const double aL = sound_r<F, R>(VL, rrL), aR = sound_r<F, R>(VR, rrR);
     const double unL = un_of<D>(VL), unR = un_of<D>(VR);
     const double pL = VL.p, pR = VR.p, rhoL = VL.r, rhoR = VR.r;
     const double SL = lmin(unL - aL, unR - aR);
@@ -224,17 +259,17 @@ __device__ __forceinline__ Cons hllc(const Cons& UL, const Prim& VL, const Cons&
     if (SR <= 0.0) return FR;
     const double numerator = pR - pL + rhoL * unL * (SL - unL) - rhoR * unR * (SR - unR);
     const double denominator = rhoL * (SL - unL) - rhoR * (SR - unR);
-    if (fabs(denominator) < 1.0e-14) return hll<D, F>(UL, VL, UR, VR);
+    if (fabs(denominator) < 1.0e-14) return hll<D, F, R>(UL, VL, UR, VR, rrL, rrR);
     const double Sstar = dv<F>(numerator, denominator);
-    if (!F && !finite(Sstar)) return hll<D, F>(UL, VL, UR, VR);   // finite in a fast tile
+    if (!F && !finite(Sstar)) return hll<D, F, R>(UL, VL, UR, VR, rrL, rrR);   // finite in a fast tile
     // One star state is formed, on the side the sign of S* picks
     // (neighbouring faces mostly agree, so the branch rarely diverges).
     if (Sstar >= 0.0) {
-        const Cons Us = hllc_star<D, F>(UL, VL, SL, Sstar);
+        const Cons Us = hllc_star<D, F, R>(UL, VL, SL, Sstar, rrL);
         return {FL.r + SL * (Us.r - UL.r), FL.mu + SL * (Us.mu - UL.mu), FL.mv + SL * (Us.mv - UL.mv),
                 FL.e + SL * (Us.e - UL.e)};
     }
-    const Cons Us = hllc_star<D, F>(UR, VR, SR, Sstar);
+    const Cons Us = hllc_star<D, F, R>(UR, VR, SR, Sstar, rrR);
     return {FR.r + SR * (Us.r - UR.r), FR.mu + SR * (Us.mu - UR.mu), FR.mv + SR * (Us.mv - UR.mv),
             FR.e + SR * (Us.e - UR.e)};
 }
@@ -242,10 +277,11 @@ __device__ __forceinline__ Cons hllc(const Cons& UL, const Prim& VL, const Cons&
 // ---------------------------------------------------------------------------
 // FORCE-type flux with local alpha (head/riemann.hpp:284-325)
 // ---------------------------------------------------------------------------
-template <int D, bool F = false>
-__device__ __forceinline__ Cons force(const Cons& UL, const Prim& VL, const Cons& UR, const Prim& VR) {
-    if (!phys_riemann<F>(VL) || !phys_riemann<F>(VR)) return hll<D, F>(UL, VL, UR, VR);
-    const double aL = sound<F>(VL), aR = sound<F>(VR);
+template <int D, bool F = false, bool R = false>
+__device__ __forceinline__ Cons force(const Cons& UL, const Prim& VL, const Cons& UR, const Prim& VR, double rrL = 0.0,
+                                      double rrR = 0.0) {
+    if (!phys_riemann<F>(VL) || !phys_riemann<F>(VR)) return hll<D, F, R>(UL, VL, UR, VR, rrL, rrR);
+    const double aL = sound_r<F, R>(VL, rrL), aR = sound_r<F, R>(VR, rrR);
     const double unL = un_of<D>(VL), unR = un_of<D>(VR);
     const double alpha = lmax(fabs(unL) + aL, fabs(unR) + aR);
     const Cons FL = pflux<D>(UL, VL);
@@ -404,11 +440,12 @@ __device__ __forceinline__ Cons exact(const Cons& UL, const Prim& VL, const Cons
 
 // F selects the unscaled division / square root for HLL, HLLC and FORCE
 // (the exact solver keeps the compiler's expansions and pow).
-template <int S, int D, bool F = false>
-__device__ __forceinline__ Cons riemann(const Cons& UL, const Prim& VL, const Cons& UR, const Prim& VR, int& iters) {
-    if (S == kHLL) return hll<D, F>(UL, VL, UR, VR);
-    if (S == kHLLC) return hllc<D, F>(UL, VL, UR, VR);
-    if (S == kForce) return force<D, F>(UL, VL, UR, VR);
+template <int S, int D, bool F = false, bool R = false>
+__device__ __forceinline__ Cons riemann(const Cons& UL, const Prim& VL, const Cons& UR, const Prim& VR, int& iters,
+                                        double rrL = 0.0, double rrR = 0.0) {
+    if (S == kHLL) return hll<D, F, R>(UL, VL, UR, VR, rrL, rrR);
+    if (S == kHLLC) return hllc<D, F, R>(UL, VL, UR, VR, rrL, rrR);
+    if (S == kForce) return force<D, F, R>(UL, VL, UR, VR, rrL, rrR);
     return exact<D>(UL, VL, UR, VR, iters);
 }
 
@@ -439,7 +476,9 @@ __device__ __forceinline__ double minmod_med(double a, double b) {
     return __builtin_fmax(__builtin_fmin(a, b), __builtin_fmin(__builtin_fmax(a, b), 0.0));
 }
 
-struct Edge { Cons U; Prim V; };
+// An evolved edge state: conserved, primitive, and (fast tiles) rcp_nr of
+// its density.
+struct Edge { Cons U; Prim V; double rr; };
 
 __device__ __forceinline__ Prim diff(const Prim& a, const Prim& b) { return {a.r - b.r, a.u - b.u, a.v - b.v, a.p - b.p}; }
 
@@ -468,17 +507,18 @@ __device__ __forceinline__ Prim reconstruct_d(const Prim& dm, const Prim& Wc, co
         }
     }
     const Cons UL = p2c<F>(WL), UR = p2c<F>(WR);
-    const Prim VL = c2p<F>(UL), VR = c2p<F>(UR);
+    double rl, rr;
+    const Prim VL = c2p_rr<F>(UL, rl), VR = c2p_rr<F>(UR, rr);
     const Cons FL = pflux<D>(UL, VL), FR = pflux<D>(UR, VR);
     const Cons h = {half * (FR.r - FL.r), half * (FR.mu - FL.mu), half * (FR.mv - FL.mv), half * (FR.e - FL.e)};
     L.U = {UL.r - h.r, UL.mu - h.mu, UL.mv - h.mv, UL.e - h.e};
     R.U = {UR.r - h.r, UR.mu - h.mu, UR.mv - h.mv, UR.e - h.e};
-    L.V = c2p<F>(L.U);
-    R.V = c2p<F>(R.U);
+    L.V = c2p_rr<F>(L.U, L.rr);
+    R.V = c2p_rr<F>(R.U, R.rr);
     const bool eL = phys_recon<F>(L.V), eR = phys_recon<F>(R.V);
     if (!__all(eL && eR)) {
-        if (!eL) { L.U = UL; L.V = VL; }
-        if (!eR) { R.U = UR; R.V = VR; }
+        if (!eL) { L.U = UL; L.V = VL; L.rr = rl; }
+        if (!eR) { R.U = UR; R.V = VR; R.rr = rr; }
     }
     return dp;
 }

@@ -33,6 +33,42 @@ __device__ __forceinline__ void st(const View& v, int i, int j, const Cons& U) {
     q[3 * v.ps] = U.e;
 }
 
+// Row access through buffer descriptors: a wavefront-uniform descriptor per
+// plane whose base is the row's cell i0 (scalar arithmetic, rebuilt per
+// row) and a per-lane 32-bit byte offset that stays in one register for the
+// whole tile, so no vector instruction forms an address.  The descriptors
+// span 2^31 bytes from the row (no range check is ever hit: offsets stay
+// within the row and its padding).
+typedef unsigned int mh_u32x2 __attribute__((ext_vector_type(2)));
+constexpr int kRsrcWord3 = 0x00020000;   // raw buffer, 32-bit data format (gfx9)
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t rsrc_at(const double* p) {
+    // every address passed here is the same in all lanes of the wavefront
+    // (a row of the tile); saying so keeps the descriptor in scalar
+    // registers where the compiler cannot prove it
+    const uint64_t a = reinterpret_cast<uint64_t>(p);
+    const uint64_t u = ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(a >> 32)) << 32) |
+                       (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)a);
+    return __builtin_amdgcn_make_buffer_rsrc(reinterpret_cast<void*>(u), (short)0, 0x7fffffff, kRsrcWord3);
+}
+__device__ __forceinline__ double bld(const double* base, uint32_t off) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rsrc_at(base), off, 0, 0));
+}
+__device__ __forceinline__ void bst(const double* base, uint32_t off, double x) {
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(mh_u32x2, x), rsrc_at(base), off, 0, 0);
+}
+__device__ __forceinline__ const double* row_at(const View& v, int i0, int j) {
+    return v.o + (int64_t)j * v.rp + i0;
+}
+__device__ __forceinline__ Cons ld_off(const View& v, const double* row, uint32_t off) {
+    return {bld(row, off), bld(row + v.ps, off), bld(row + 2 * v.ps, off), bld(row + 3 * v.ps, off)};
+}
+__device__ __forceinline__ void st_off(const View& v, const double* row, uint32_t off, const Cons& U) {
+    bst(row, off, U.r);
+    bst(row + v.ps, off, U.mu);
+    bst(row + 2 * v.ps, off, U.mv);
+    bst(row + 3 * v.ps, off, U.e);
+}
+
 // Every lane of a tile issues its stores, also a lane with no cell to write:
 // that lane writes into column kSinkCol of the row, which lies in the row's
 // lead padding (left of the two ghost columns) and is never read.  With no
@@ -114,13 +150,25 @@ __device__ __forceinline__ bool dt_in_fast_range(double dtd) { return dtd >= 0x1
 // cell and ghost the fast pass stored, and the fast pass's speed maximum is
 // dropped.  The IEEE pass stays out of the fast pass's loop (and, for the
 // single-grid sweep, out of its kernel), which keeps the hot loop lean.
-template <int S, bool F>
+// B: rows through buffer descriptors and the Riemann solve on the edge
+// states' shared reciprocals (the single-grid sweeps); without it, 64-bit
+// lane addresses and the solver's own divisions (the batch kernels, whose
+// register budget the shared reciprocals exceed).  The results are the same.
+template <int S, bool F, bool B = true>
 __device__ __forceinline__ bool y_tile_pass(const View& in, const View& out, int cg, int j0, int j1, double dtd,
                                             double half, const Bc& bc, int ghosts, int speed, double* ring, int lane,
                                             double& m) {
     const int i = cg * 64 + lane;
     const bool active = i < in.nx;
     const int ic = active ? i : in.nx - 1;
+    // loads: cell ic of the row from column 0; stores: the lane's cell, or
+    // the sink column for a lane past the row's end, from the sink column
+    const uint32_t lo = (uint32_t)ic * (uint32_t)sizeof(double);
+    const uint32_t so = (uint32_t)(active ? i - kSinkCol : 0) * (uint32_t)sizeof(double);
+    auto load_row = [&](int r) {
+        if constexpr (B) return ld_off(in, row_at(in, 0, r), lo);
+        else return ld(in, ic, r);
+    };
     int it = 0;
     auto put = [&](int slot, const Cons& U) {
         double* q = ring + slot * 256 + lane;
@@ -143,19 +191,19 @@ __device__ __forceinline__ bool y_tile_pass(const View& in, const View& out, int
     // rows j0-2 and j0-1: convert only.  The window carries W of row p-1
     // and the difference dm = W(p-1) - W(p-2), which is all the limiter
     // takes of row p-2.
-    Cons U0 = ld(in, ic, j0 - 2);
+    Cons U0 = load_row(j0 - 2);
     Prim dm, W1, W0;
     conv(U0, W0);
     put(0, U0);
-    U0 = ld(in, ic, j0 - 1);
+    U0 = load_row(j0 - 1);
     conv(U0, W1);
     put(1, U0);
     dm = diff(W1, W0);
-    Cons Un = ld(in, ic, j0);
+    Cons Un = load_row(j0);
     Edge L, R, Rp;
     {   // row p = j0: reconstruct j0-1
         U0 = Un;
-        Un = ld(in, ic, j0 + 1);
+        Un = load_row(j0 + 1);
         conv(U0, W0);
         put(2, U0);
         dm = reconstruct_d<kY, F>(dm, W1, W0, half, L, Rp);
@@ -164,11 +212,11 @@ __device__ __forceinline__ bool y_tile_pass(const View& in, const View& out, int
     Cons Fp;
     {   // row p = j0+1: reconstruct j0, face j0-1/2
         U0 = Un;
-        Un = ld(in, ic, j0 + 2);
+        Un = load_row(j0 + 2);
         conv(U0, W0);
         put(0, U0);   // row j0-2 is no longer needed
         dm = reconstruct_d<kY, F>(dm, W1, W0, half, L, R);
-        Fp = riemann<S, kY, F>(Rp.U, Rp.V, L.U, L.V, it);
+        Fp = riemann<S, kY, F, B>(Rp.U, Rp.V, L.U, L.V, it, Rp.rr, L.rr);
         Rp = R;
         W1 = W0;
     }
@@ -179,12 +227,13 @@ __device__ __forceinline__ bool y_tile_pass(const View& in, const View& out, int
         // the next row's load, so that load may reuse Un's registers.
         conv(Un, W0);
         put(slot_new, Un);
-        Un = ld(in, ic, min(p + 1, j1 + 1));
+        Un = load_row(min(p + 1, j1 + 1));
         dm = reconstruct_d<kY, F>(dm, W1, W0, half, L, R);
-        const Cons Fc = riemann<S, kY, F>(Rp.U, Rp.V, L.U, L.V, it);
+        const Cons Fc = riemann<S, kY, F, B>(Rp.U, Rp.V, L.U, L.V, it, Rp.rr, L.rr);
         const int r = p - 2;
         const Cons Unew = update(get(slot_old), dtd, Fp, Fc);
-        st(out, active ? i : kSinkCol, r, Unew);
+        if constexpr (B) st_off(out, row_at(out, kSinkCol, r), so, Unew);
+        else st(out, active ? i : kSinkCol, r, Unew);
         if (active) {
             // From rows in range, the new state's rho and p are zero or within
             // about 2^-150..2^400 (differences of in-range values cannot fall
@@ -225,13 +274,14 @@ __device__ __forceinline__ bool y_tile_pass(const View& in, const View& out, int
 // column, so program order is the only ordering needed); `ring` is the
 // wavefront's kRing-double LDS slice.  The fast pass runs first when dt/d is
 // in range (see y_tile_pass).
-template <int S>
+template <int S, bool B = true>
 __device__ __forceinline__ double y_tile(const View& in, const View& out, int cg, int j0, int j1, double dtd,
                                          double half, const Bc& bc, int ghosts, int speed, double* ring, int lane) {
     double m = 0.0;
-    if (dt_in_fast_range(dtd) && y_tile_pass<S, true>(in, out, cg, j0, j1, dtd, half, bc, ghosts, speed, ring, lane, m))
+    if (dt_in_fast_range(dtd) &&
+        y_tile_pass<S, true, B>(in, out, cg, j0, j1, dtd, half, bc, ghosts, speed, ring, lane, m))
         return m;
-    y_tile_pass<S, false>(in, out, cg, j0, j1, dtd, half, bc, ghosts, speed, ring, lane, m);
+    y_tile_pass<S, false, B>(in, out, cg, j0, j1, dtd, half, bc, ghosts, speed, ring, lane, m);
     return m;
 }
 
@@ -396,7 +446,8 @@ __device__ __forceinline__ bool x_tilec_pass(const View& in, const View& out, in
 // across the next iteration's conversion only.  The U slots hold the
 // previous iteration's states until that read is done, so lanes 62-63 copy
 // theirs into the carry before every lane writes its new one.
-template <int S, bool F>
+// B as for y_tile_pass.
+template <int S, bool F, bool B = true>
 __device__ __forceinline__ bool x_tiled_pass(const View& in, const View& out, int j, int i0, int i1, double dtd,
                                              double half, const Bc& bc, int ghosts, XLdsC& X, int lane) {
     int it = 0;
@@ -404,12 +455,18 @@ __device__ __forceinline__ bool x_tiled_pass(const View& in, const View& out, in
     const bool l1 = lane >= 1, l2 = lane >= 2;
     unsigned key = 0;   // running range key of the states read (fast pass)
     Cons Fprev = {0.0, 0.0, 0.0, 0.0};
-    // the update of the cell two left of lane `lane`'s cell of iteration kk
+    const uint32_t lane8 = (uint32_t)lane * (uint32_t)sizeof(double);
+    // the update of the cell two left of lane `lane`'s cell of iteration kk:
+    // stores address cell i0-4+64kk of the row plus the lane's offset; a
+    // lane with no cell of the segment gets an offset past the descriptor's
+    // range, and the hardware drops its store (every lane still issues it)
     auto update_prev = [&](int kk, const Cons& Fm, const Cons& U2) {
-        const int r = i0 - 4 + 64 * kk + lane;
-        const bool mine = r >= i0 && r < i1;
+        const int c0 = i0 - 4 + 64 * kk;
+        const int r = c0 + lane;
+        const bool mine = (unsigned)(lane - (i0 - c0)) < (unsigned)(i1 - i0);
         const Cons Unew = update(U2, dtd, Fm, Fprev);
-        st(out, mine ? r : kSinkCol, j, Unew);
+        if constexpr (B) st_off(out, row_at(out, c0, j), mine ? lane8 : 0x80000000u, Unew);
+        else st(out, mine ? r : kSinkCol, j, Unew);
         if (mine && ghosts) {
             if (j == 0) {
                 const Cons G = bc.kind[MH_EDGE_BOTTOM] == MH_BC_INFLOW ? bc.s[MH_EDGE_BOTTOM] : Unew;
@@ -434,11 +491,22 @@ __device__ __forceinline__ bool x_tiled_pass(const View& in, const View& out, in
         Fm = {f1[0], f1[1], f1[2], f1[3]};
         U2 = {u2[0], u2[1], u2[2], u2[3]};
     };
-    Cons Un = ld(in, min(i0 - 2 + lane, in.nx + 1), j);
+    // loads: the pass's cells i0-2+64k+lane, clamped to the last ghost cell
+    // nx+1 -- the row address of cell min(c0, nx+1) and the lane's offset up
+    // to that ghost
+    auto load_pass = [&](int c0) {
+        if constexpr (B) {
+            const int cb = min(c0, in.nx + 1);
+            const uint32_t lim = (uint32_t)(in.nx + 1 - cb) * (uint32_t)sizeof(double);
+            return ld_off(in, row_at(in, cb, j), min(lane8, lim));
+        } else {
+            return ld(in, min(c0 + lane, in.nx + 1), j);
+        }
+    };
+    Cons Un = load_pass(i0 - 2);
     for (int k = 0; k < niter; ++k) {
-        const int p = i0 - 2 + 64 * k + lane;
         const Cons U0 = Un;
-        Un = ld(in, min(p + 64, in.nx + 1), j);
+        Un = load_pass(i0 - 2 + 64 * (k + 1));
         const Prim W0 = c2p<F>(U0);
         if constexpr (F) key = max(key, range_key(U0.r, U0.e, W0.p));
         X.M[0][lane] = W0.r; X.M[1][lane] = W0.u; X.M[2][lane] = W0.v; X.M[3][lane] = W0.p;
@@ -463,19 +531,24 @@ __device__ __forceinline__ bool x_tiled_pass(const View& in, const View& out, in
         const Prim W2 = {q2[0], q2[1], q2[2], q2[3]};
         Edge L, R;
         reconstruct<kX, F>(W2, W1, W0, half, L, R);
+        // the right edge state to the lane on the right: conserved state,
+        // u, v, p, and in the slot of its density (which equals the
+        // conserved density) rcp_nr of the density for the fast tile's
+        // Riemann solve
+        const double r4 = (F && B) ? R.rr : R.V.r;
         X.M[0][lane] = R.U.r; X.M[1][lane] = R.U.mu; X.M[2][lane] = R.U.mv; X.M[3][lane] = R.U.e;
-        X.M[4][lane] = R.V.r; X.M[5][lane] = R.V.u; X.M[6][lane] = R.V.v; X.M[7][lane] = R.V.p;
+        X.M[4][lane] = r4; X.M[5][lane] = R.V.u; X.M[6][lane] = R.V.v; X.M[7][lane] = R.V.p;
         wave_sync();
         double r1[8];
         for (int c = 0; c < 8; ++c) r1[c] = l1 ? X.M[c][lane - 1] : X.cR[c];
         wave_sync();
         if (lane == 63) {
             X.cR[0] = R.U.r; X.cR[1] = R.U.mu; X.cR[2] = R.U.mv; X.cR[3] = R.U.e;
-            X.cR[4] = R.V.r; X.cR[5] = R.V.u; X.cR[6] = R.V.v; X.cR[7] = R.V.p;
+            X.cR[4] = r4; X.cR[5] = R.V.u; X.cR[6] = R.V.v; X.cR[7] = R.V.p;
         }
         const Cons RU = {r1[0], r1[1], r1[2], r1[3]};
-        const Prim RV = {r1[4], r1[5], r1[6], r1[7]};
-        Fprev = riemann<S, kX, F>(RU, RV, L.U, L.V, it);
+        const Prim RV = {r1[0], r1[5], r1[6], r1[7]};
+        Fprev = riemann<S, kX, F, B>(RU, RV, L.U, L.V, it, r1[4], L.rr);
         X.M[4][lane] = Fprev.r; X.M[5][lane] = Fprev.mu; X.M[6][lane] = Fprev.mv; X.M[7][lane] = Fprev.e;
     }
     wave_sync();
@@ -492,7 +565,7 @@ __device__ __forceinline__ bool x_tiled_pass(const View& in, const View& out, in
 template <int L> struct XSlice { using type = XLdsC; };
 template <> struct XSlice<1> { using type = XLds66; };
 
-template <int S, int L>
+template <int S, int L, bool B = true>
 __device__ __forceinline__ void x_tile(const View& in, const View& out, int j, int i0, int i1, double dtd,
                                        double half, const Bc& bc, int ghosts, typename XSlice<L>::type& X,
                                        int lane) {
@@ -502,8 +575,9 @@ __device__ __forceinline__ void x_tile(const View& in, const View& out, int j, i
         if (dt_in_fast_range(dtd) && x_tile66_pass<S, true>(in, out, j, i0, i1, dtd, half, bc, ghosts, X, lane)) return;
         x_tile66_pass<S, false>(in, out, j, i0, i1, dtd, half, bc, ghosts, X, lane);
     } else if constexpr (L == 2) {
-        if (dt_in_fast_range(dtd) && x_tiled_pass<S, true>(in, out, j, i0, i1, dtd, half, bc, ghosts, X, lane)) return;
-        x_tiled_pass<S, false>(in, out, j, i0, i1, dtd, half, bc, ghosts, X, lane);
+        if (dt_in_fast_range(dtd) && x_tiled_pass<S, true, B>(in, out, j, i0, i1, dtd, half, bc, ghosts, X, lane))
+            return;
+        x_tiled_pass<S, false, B>(in, out, j, i0, i1, dtd, half, bc, ghosts, X, lane);
     } else {
         if (dt_in_fast_range(dtd) && x_tilec_pass<S, true>(in, out, j, i0, i1, dtd, half, bc, ghosts, X, lane)) return;
         x_tilec_pass<S, false>(in, out, j, i0, i1, dtd, half, bc, ghosts, X, lane);

@@ -10,10 +10,11 @@ Each instance is a side x side grid on [0,2]^2 with the blast centred at
 (1,1): rho = 1, p = p_in for r <= r0, rho = 0.125, p = 0.1 outside,
 transmissive boundaries, CFL 0.8.
 
-``Batch`` drives the persistent batch kernel (csrc/batch.hip) through the C
-ABI; ``autotune`` times candidate schedules (tile shapes, one fused launch or
-two launches per step, workgroups per CU) on the batch itself and keeps the
-fastest.
+``Batch`` drives the batch kernels (csrc/batch.hip) through the C ABI;
+``autotune`` times candidate schedules (tile shapes, the persistent kernel
+with one or two launches per step or dispatched launches, workgroups per CU)
+and then size classes (smaller instances with their own tile shapes) on the
+batch itself and keeps the fastest.
 """
 from __future__ import annotations
 
@@ -70,6 +71,7 @@ class Batch:
         call("mh_batch_create", self.n, descs, self.solver, float(cfl), None, C.byref(self._h))
         self.cells = sum(s["side"] * s["side"] for s in self.spec)
         self.schedule = (0, 256, 0, 0, 1)
+        self.size_class = (0, 0, 0)
 
     def grid(self, k: int) -> Grid2DGPU:
         """A Grid2DGPU-like view of instance k (non-owning)."""
@@ -96,6 +98,12 @@ class Batch:
         one tile per wavefront."""
         call("mh_batch_set_schedule", self._h, int(xseg), int(xrows), int(ychunk), int(mode), int(blocks_per_cu))
         self.schedule = (int(xseg), int(ychunk), int(mode), int(blocks_per_cu), int(xrows))
+
+    def set_size_class(self, side_below=0, ychunk=0, xrows=0):
+        """Instances with fewer than side_below cells a side take y chunks of
+        ychunk rows and x tiles of xrows rows (0: the schedule's)."""
+        call("mh_batch_set_size_class", self._h, int(side_below), int(ychunk), int(xrows))
+        self.size_class = (int(side_below), int(ychunk), int(xrows))
 
     def capture(self, steps_per_graph: int, stream=None):
         call("mh_batch_capture", self._h, int(steps_per_graph), _stream(stream))
@@ -130,6 +138,11 @@ class Batch:
             pass
 
 
+# Size classes tried on the best dispatched schedule: (side below which an
+# instance takes them, y chunk rows, x tile rows); (0, 0, 0) is none.
+SIZE_CLASSES = [(0, 0, 0)] + [(side, yc, xr) for side in (256, 512)
+                              for yc, xr in ((32, 8), (64, 4), (64, 8), (128, 8), (32, 16), (256, 8), (0, 16))]
+
 DEFAULT_CANDIDATES = [
     # (xseg, ychunk, mode, blocks_per_cu, xrows)
     (0, 256, 0, 0, 1), (0, 128, 1, 0, 1), (0, 256, 2, 0, 4), (0, 192, 2, 0, 4), (0, 128, 2, 0, 4),
@@ -142,47 +155,71 @@ def _signature(batch: Batch) -> str:
     return f"n{batch.n}-cells{batch.cells}-s{batch.solver}-min{sides[0]}-max{sides[-1]}"
 
 
-def autotune(batch: Batch, candidates=None, steps: int = 8, cache: str | None = None, log=None, rounds: int = 2):
+def _time(batch, steps, set_up):
+    set_up()
+    batch.initialise()
+    batch.step(1)
+    batch.clock()
+    t0 = time.perf_counter()
+    batch.step(steps)
+    batch.clock()
+    return (time.perf_counter() - t0) / steps
+
+
+def autotune(batch: Batch, candidates=None, steps: int = 8, cache: str | None = None, log=None, rounds: int = 2,
+             size_classes=None):
     """Times each candidate schedule for `steps` steps (after one untimed
-    step), in `rounds` interleaved rounds, and applies the one with the
-    fastest best round (one short sample per candidate let timing noise pick
-    a schedule 9 % slower than the best); re-initialises the batch
-    afterwards.  Results are cached per ensemble signature in `cache` (JSON)."""
+    step), in `rounds` interleaved rounds, and keeps the one with the fastest
+    best round (one short sample per candidate let timing noise pick a
+    schedule 9 % slower than the best); where that is a dispatched schedule,
+    then times the size classes (SIZE_CLASSES: smaller instances with their
+    own tile shapes) on it the same way.  Applies the winner and
+    re-initialises the batch.  Results are cached per ensemble signature in
+    `cache` (JSON)."""
     sig = _signature(batch)
     if cache and os.path.exists(cache):
         with open(cache) as f:
             known = json.load(f)
         if sig in known:
             best = tuple(known[sig]["schedule"])
+            cls = tuple(known[sig].get("size_class", (0, 0, 0)))
             batch.set_schedule(*best)
+            batch.set_size_class(*cls)
             batch.initialise()
-            return {"schedule": best, "cached": True, "timings": known[sig].get("timings")}
+            return {"schedule": best, "size_class": cls, "cached": True, "timings": known[sig].get("timings")}
     cands = list(candidates or DEFAULT_CANDIDATES)
     best_dt = {}
+    batch.set_size_class(0, 0, 0)
     for _ in range(max(1, rounds)):
         for cand in cands:
-            batch.set_schedule(*cand)
-            batch.initialise()
-            batch.step(1)
-            batch.clock()
-            t0 = time.perf_counter()
-            batch.step(steps)
-            batch.clock()
-            dt = (time.perf_counter() - t0) / steps
+            dt = _time(batch, steps, lambda: batch.set_schedule(*cand))
             best_dt[cand] = min(dt, best_dt.get(cand, dt))
             if log:
                 log(f"autotune {cand}: {dt * 1e3:.3f} ms/step")
     timings = sorted((t, c) for c, t in best_dt.items())
     best = timings[0][1]
+    cls_dt = {(0, 0, 0): timings[0][0]}
+    if best[2] == 2:
+        batch.set_schedule(*best)
+        for _ in range(max(1, rounds)):
+            for cls in (size_classes or SIZE_CLASSES):
+                dt = _time(batch, steps, lambda: batch.set_size_class(*cls))
+                cls_dt[cls] = min(dt, cls_dt.get(cls, dt))
+                if log:
+                    log(f"autotune size class {cls}: {dt * 1e3:.3f} ms/step")
+    cls_best = min((t, c) for c, t in cls_dt.items())[1]
     batch.set_schedule(*best)
+    batch.set_size_class(*cls_best)
     batch.initialise()
-    res = {"schedule": best, "cached": False, "timings": [(c, t) for t, c in timings]}
+    res = {"schedule": best, "size_class": cls_best, "cached": False, "timings": [(c, t) for t, c in timings],
+           "size_class_timings": sorted(((c, t) for c, t in cls_dt.items()), key=lambda x: x[1])}
     if cache:
         known = {}
         if os.path.exists(cache):
             with open(cache) as f:
                 known = json.load(f)
-        known[sig] = {"schedule": list(best), "timings": [[list(c), t] for t, c in timings]}
+        known[sig] = {"schedule": list(best), "size_class": list(cls_best),
+                      "timings": [[list(c), t] for t, c in timings]}
         os.makedirs(os.path.dirname(cache) or ".", exist_ok=True)
         with open(cache, "w") as f:
             json.dump(known, f, indent=1)

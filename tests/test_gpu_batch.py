@@ -45,6 +45,28 @@ def test_batch_instances_match_oracle(hip, oracle, schedule):
     b.close()
 
 
+@pytest.mark.parametrize("schedule, size_class", [((0, 64, 2, 0, 4), (100, 16, 3)), ((0, 256, 0, 0, 1), (1000, 0, 5)),
+                                                  ((60, 128, 2, 0, 8), (50, 40, 0))])
+def test_batch_size_class_matches_oracle(hip, oracle, schedule, size_class):
+    """Instances below a side take their own y chunks and x tile rows (the
+    autotuner's size classes): every instance still equals its own oracle
+    run bit for bit."""
+    from large_sys_eq_opt_amd.ensemble import Batch
+
+    b = Batch(SPEC, solver=1)
+    b.set_schedule(*schedule)
+    b.set_size_class(*size_class)
+    b.initialise()
+    b.step(STEPS)
+    t, dt, steps, status = b.clock()
+    assert (steps == STEPS).all() and (status == 0).all()
+    for k, s in enumerate(SPEC):
+        o = oracle_instance(oracle, s, 1, STEPS)
+        assert same_bits(interior(b.download(k)), interior(o["U"])), (k, s)
+        assert dt[k] == o["dt"][-1] and t[k] == np.float64(sum_seq(o["dt"]))
+    b.close()
+
+
 def sum_seq(dts):
     t = 0.0
     for d in dts:

@@ -2,8 +2,9 @@
 compute_dt_gpu / advance_*_gpu API over libmh_euler.so, and the
 device-resident loop) compiles against include/mh_euler.h and the
 reference's own headers (head/types.hpp, head/riemann.hpp) and links
-against the library.  Nothing runs (no GPU here); skipped where the
-reference tree is absent."""
+against the library; the driver built on it (tests/integration_driver.py)
+is what tests/test_gpu_integration.py runs on the GPU.  Nothing runs here
+(no GPU); skipped where the reference tree is absent."""
 from __future__ import annotations
 
 import os
@@ -59,3 +60,14 @@ int main() {{
            "-o", str(tmp_path / "main")]
     r = subprocess.run(cmd, capture_output=True, text=True)
     assert r.returncode == 0, r.stderr[-3000:]
+
+
+@pytest.mark.skipif(not os.path.isdir(REF_HEAD) or shutil.which("g++") is None,
+                    reason="needs the reference headers and g++")
+def test_integration_driver_builds():
+    import integration_driver as drv
+
+    if not os.path.exists(os.path.join(ROOT, "large_sys_eq_opt_amd", "libmh_euler.so")):
+        pytest.skip("library not built")
+    exe = drv.build()
+    assert exe and os.access(exe, os.X_OK)
