@@ -288,6 +288,15 @@ int mh_synchronize(mh_stream stream);
  * faces of AoS conserved states.  iters (may be NULL) receives the exact
  * solver's Newton iteration count per face (0 for the other solvers). */
 int mh_riemann_flux(int n, const double* UL, const double* UR, int dir, int solver, double* F, int* iters);
+/* No reference counterpart.  mh_riemann_flux for callers that cannot
+ * allocate device memory (code in tight serving loops, harnesses that forbid
+ * hipMalloc): the same results, staged through a fixed device array of the
+ * library (6.5 MB per device, held until the process ends) instead of
+ * per-call allocations; n <= MH_NOALLOC_MAX_FACES, larger n is refused with
+ * MH_ERR_INVALID.  Calls on one device share that array: calls that overlap
+ * on one GPU (several host threads) race. */
+#define MH_NOALLOC_MAX_FACES 65536
+int mh_riemann_flux_noalloc(int n, const double* UL, const double* UR, int dir, int solver, double* F, int* iters);
 /* No reference counterpart: the sweeps' fast arithmetic pointwise, for tests
  * that pin it to the compiler's (host buffers of n values; U is n AoS
  * states).  fast = 1: q = a/b and r = sqrt(a) by the unscaled Newton

@@ -99,6 +99,7 @@ SIGNATURES = {
     "mh_synchronize": ([_P], _I),
     "mh_fast_ops": ([_I, _P, _P, _P, _I, _P, _P, _P], _I),
     "mh_riemann_flux": ([_I, _P, _P, _I, _I, _P, _P], _I),
+    "mh_riemann_flux_noalloc": ([_I, _P, _P, _I, _I, _P, _P], _I),
     "mh_reconstruct": ([_I, _P, _P, _P, _D, _I, _P, _P], _I),
 }
 

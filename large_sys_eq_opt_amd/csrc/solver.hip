@@ -1070,6 +1070,81 @@ int mh_riemann_flux(int n, const double* UL, const double* UR, int dir, int solv
     return rc;
 }
 
+// ---- mh_riemann_flux_noalloc: the same evaluation staged through a fixed
+// device array instead of per-call allocations ------------------------------
+}  // extern "C"
+
+namespace mh {
+namespace {
+constexpr int kNoAllocMaxFaces = MH_NOALLOC_MAX_FACES;
+// Staging for mh_riemann_flux_noalloc: UL, UR, F (4 doubles per face each),
+// then the Newton iteration counts (ints), for at most kNoAllocMaxFaces
+// faces: 12 * 2^16 doubles + 2^16 ints = 6.5 MB per device, held until the
+// process ends.  In host code `scratch` names a host-side stand-in; its
+// device address comes from hipGetSymbolAddress, per device.
+static __device__ __align__(256) double scratch[12 * kNoAllocMaxFaces + kNoAllocMaxFaces / 2];
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_lock;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// The calling thread's device's copy of `scratch`, looked up once per device.
+int scratch_ptr(double** out) {
+    int dev = 0;
+    int rc = check_hip(hipGetDevice(&dev), "hipGetDevice");
+    if (rc) return rc;
+    if (dev < 0 || dev >= kMaxDevices) return fail(MH_ERR_INVALID, "mh_riemann_flux_noalloc: device index out of range");
+    std::lock_guard<std::mutex> lk(g_scratch_lock);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        rc = check_hip(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+        if (rc) return rc;
+        g_scratch_ptr[dev] = p;
+    }
+    *out = static_cast<double*>(g_scratch_ptr[dev]);
+    return MH_OK;
+}
+}  // namespace
+}  // namespace mh
+
+extern "C" {
+
+int mh_riemann_flux_noalloc(int n, const double* UL, const double* UR, int dir, int solver, double* F, int* iters) {
+    if (n < 0 || (n > 0 && (!UL || !UR || !F))) return fail(MH_ERR_INVALID, "mh_riemann_flux_noalloc: bad argument");
+    if (n > kNoAllocMaxFaces)
+        return fail(MH_ERR_INVALID, "mh_riemann_flux_noalloc: n exceeds MH_NOALLOC_MAX_FACES (" +
+                                        std::to_string(kNoAllocMaxFaces) + "); use mh_riemann_flux");
+    if (dir != 0 && dir != 1) return fail(MH_ERR_INVALID, "mh_riemann_flux_noalloc: dir must be 0 or 1");
+    if (solver < 0 || solver > 3) return fail(MH_ERR_INVALID, "mh_riemann_flux_noalloc: unknown solver");
+    if (n == 0) return MH_OK;
+    double* base = nullptr;
+    int rc = scratch_ptr(&base);
+    if (rc) return rc;
+    // Every face slot below n is written by the kernel before it is read
+    // back, so nothing needs clearing between calls.
+    double* dL = base;
+    double* dR = base + 4 * (size_t)kNoAllocMaxFaces;
+    double* dF = base + 8 * (size_t)kNoAllocMaxFaces;
+    int* dI = reinterpret_cast<int*>(base + 12 * (size_t)kNoAllocMaxFaces);
+    const size_t bytes = sizeof(double) * 4 * (size_t)n;
+    rc = check_hip(hipMemcpy(dL, UL, bytes, hipMemcpyHostToDevice), "hipMemcpy");
+    if (!rc) rc = check_hip(hipMemcpy(dR, UR, bytes, hipMemcpyHostToDevice), "hipMemcpy");
+    if (rc) return rc;
+    const dim3 g((n + 255) / 256), blk(256);
+#define MH_FF(S_, D_) hipLaunchKernelGGL((k_face_flux<S_, D_>), g, blk, 0, 0, n, dL, dR, dF, dI)
+    if (dir == 0) {
+        switch (solver) { case 0: MH_FF(kHLL, kX); break; case 1: MH_FF(kHLLC, kX); break;
+                          case 2: MH_FF(kExact, kX); break; default: MH_FF(kForce, kX); }
+    } else {
+        switch (solver) { case 0: MH_FF(kHLL, kY); break; case 1: MH_FF(kHLLC, kY); break;
+                          case 2: MH_FF(kExact, kY); break; default: MH_FF(kForce, kY); }
+    }
+#undef MH_FF
+    rc = check_hip(hipGetLastError(), "face flux launch");
+    if (!rc) rc = check_hip(hipMemcpy(F, dF, bytes, hipMemcpyDeviceToHost), "hipMemcpy");
+    if (!rc && iters) rc = check_hip(hipMemcpy(iters, dI, sizeof(int) * (size_t)n, hipMemcpyDeviceToHost), "hipMemcpy");
+    return rc;
+}
+
 int mh_reconstruct(int n, const double* Um, const double* Uc, const double* Up, double dt_over_d, int dir, double* UL,
                    double* UR) {
     if (n < 0 || (n > 0 && (!Um || !Uc || !Up || !UL || !UR))) return fail(MH_ERR_INVALID, "mh_reconstruct: bad argument");
