@@ -17,6 +17,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <charconv>
 #include <cstdio>
 #include <cstring>
 #include <new>
@@ -29,16 +30,18 @@
 namespace mh {
 namespace {
 
-// Appends "%.{prec}g" of v (what an ostream with that precision prints).
+// Appends "%.{prec}g" of v (what an ostream with that precision prints):
+// std::to_chars in general format with a precision is specified to give
+// printf's %.{prec}g output, without printf's per-call format parsing.
 inline void put_g(std::string& o, double v, int prec) {
     char b[40];
-    const int n = std::snprintf(b, sizeof b, "%.*g", prec, v);
-    o.append(b, (size_t)n);
+    const std::to_chars_result r = std::to_chars(b, b + sizeof b, v, std::chars_format::general, prec);
+    o.append(b, (size_t)(r.ptr - b));
 }
 inline void put_i(std::string& o, int v) {
     char b[16];
-    const int n = std::snprintf(b, sizeof b, "%d", v);
-    o.append(b, (size_t)n);
+    const std::to_chars_result r = std::to_chars(b, b + sizeof b, v);
+    o.append(b, (size_t)(r.ptr - b));
 }
 
 // Cell (i, j) of the (ny+4) x (nx+4) ghosted array, conserved component c.
