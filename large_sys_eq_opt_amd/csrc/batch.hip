@@ -134,6 +134,17 @@ __device__ __forceinline__ void commit_clocks(const BatchArgs& a, int par) {
     }
 }
 
+// One x tile: whole rows as one stream of passes (x_stream), segments of a
+// row one row at a time.
+template <int S>
+__device__ __forceinline__ void x_phase_tile(const BatchInst& I, const XTile& t, double dtd, XLdsC& X, int lane) {
+    if (t.i0 == 0 && t.i1 == I.U.nx) {
+        x_stream<S>(I.U, I.T, t.j0, t.j1, dtd, 0.5 * dtd, I.bc, 1, X, lane);
+        return;
+    }
+    for (int j = t.j0; j < t.j1; ++j) x_tile<S, 2, false>(I.U, I.T, j, t.i0, t.i1, dtd, 0.5 * dtd, I.bc, 1, X, lane);
+}
+
 union BatchSlice {   // a wavefront's LDS: x-tile exchange or y-tile ring
     XLdsC x;
     double ring[kRing];
@@ -159,9 +170,7 @@ __global__ __launch_bounds__(256) void k_batch_step(BatchArgs a) {
             const BatchInst& I = a.inst[t.inst];
             double dt;
             if (!inst_dt(I, a.clk[t.inst], par, dt)) continue;
-            const double dtd = dt / I.dx;
-            for (int j = t.j0; j < t.j1; ++j)
-                x_tile<S, 2, false>(I.U, I.T, j, t.i0, t.i1, dtd, 0.5 * dtd, I.bc, 1, X[w].x, lane);
+            x_phase_tile<S>(I, t, dt / I.dx, X[w].x, lane);
         }
     }
     // phase y rewrites U: never on top of an incomplete x phase
@@ -200,11 +209,12 @@ __global__ __launch_bounds__(64) void k_batch_x(BatchArgs a, int ntiles) {
     const int q = blockIdx.x;
     if (q >= ntiles) return;
     const XTile t = a.xt[q];
+    // the instance's fields are read once here, before any store (the x
+    // tile's buffer stores could alias them and force reloads every pass)
     const BatchInst& I = a.inst[t.inst];
     double dt;
     if (!inst_dt(I, a.clk[t.inst], par, dt)) return;
-    const double dtd = dt / I.dx;
-    for (int j = t.j0; j < t.j1; ++j) x_tile<S, 2, false>(I.U, I.T, j, t.i0, t.i1, dtd, 0.5 * dtd, I.bc, 1, X[w], lane);
+    x_phase_tile<S>(I, t, dt / I.dx, X[w], lane);
 }
 
 // Dispatched y phase: wavefront q runs y tile q with the fast division
@@ -388,13 +398,18 @@ int build_schedule(mh_batch b) {
         const int nx = b->hinst[k].U.nx, ny = b->hinst[k].U.ny;
         const bool small = b->small_side > 0 && nx < b->small_side;
         const int seg = b->xseg > 0 ? b->xseg : nx;
-        const int rows = std::max(1, small && b->small_xrows > 0 ? b->small_xrows : b->xrows);
+        // whole-row tiles are streamed (x_stream): their rows must span less
+        // than 2^31 bytes of a plane
+        const int64_t rp = b->hinst[k].U.rp;
+        const int rows = (int)std::min<int64_t>(std::max(1, small && b->small_xrows > 0 ? b->small_xrows : b->xrows),
+                                                std::max<int64_t>(1, ((int64_t)1 << 28) / rp - 1));
         const int ychunk = small && b->small_ychunk > 0 ? b->small_ychunk : b->ychunk;
         for (int j0 = 0; j0 < ny; j0 += rows)
             for (int i0 = 0; i0 < nx; i0 += seg) {
                 const int i1 = std::min(nx, i0 + seg), j1 = std::min(ny, j0 + rows);
                 xt.push_back({k, j0, j1, i0, i1});
-                xc.push_back(64.0 * (j1 - j0) * ((i1 - i0 + 4 + 63) / 64));
+                xc.push_back(seg == nx ? 64.0 * (((int64_t)(j1 - j0) * (nx + 4) + 63) / 64)
+                                       : 64.0 * (j1 - j0) * ((i1 - i0 + 4 + 63) / 64));
             }
         const int ch = ychunk > 0 ? ychunk : ny;
         for (int cg = 0; cg * 64 < nx; ++cg)

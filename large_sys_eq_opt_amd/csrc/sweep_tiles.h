@@ -560,6 +560,178 @@ __device__ __forceinline__ bool x_tiled_pass(const View& in, const View& out, in
     return true;
 }
 
+// XStream: whole rows j0..j1-1 as one stream of rows of nx+4 cells (each
+// row with its two ghost cells on either side), 64 stream positions per
+// pass, so one pass may hold the end of a row and the start of the next.  A
+// cell's update reads only cells c-2..c+2 of its own row, so every update
+// stored is the one the row alone gives; the positions whose reconstruction
+// or faces mix two rows are ghost cells, whose updates are not stored.
+// Against one segment per row, short rows leave no lanes idle (a row of 64
+// cells takes 68 lanes, not 128) and a row's first loads are in flight
+// while the row before it ends.  Rows through buffer descriptors at cell
+// (-2, j0) of each grid; in and out have the same pitch and plane size, and
+// (j1-j0) rows of them span less than 2^31 bytes (the host keeps to this).
+struct XStream {
+    // by value: the buffer stores below could alias fields read through a
+    // reference, which would then be reloaded every pass
+    View in, out;
+    const Bc& bc;
+    int j0, j1, ghosts, lane;
+    double dtd;
+    int w, len;              // stream row length nx+4, positions (j1-j0)*w
+    int wraps;               // most row ends one pass can cross
+    uint32_t olast, wrapb;   // byte offsets: position len-1; row change
+    const double* inb;
+    const double* outb;
+    // load and store cursors: cell index and byte offset of the lane's
+    // position in the next pass to load / store
+    int lc, sc;
+    uint32_t lo, so;
+    __device__ XStream(const View& in_, const View& out_, const Bc& bc_, int j0_, int j1_, int ghosts_, int lane_,
+                       double dtd_)
+        : in(in_), out(out_), bc(bc_), j0(j0_), j1(j1_), ghosts(ghosts_), lane(lane_), dtd(dtd_) {
+        w = in.nx + 4;
+        len = (j1 - j0) * w;
+        wraps = w >= 64 ? 1 : 64 / w + 1;
+        olast = (uint32_t)((int64_t)(j1 - j0 - 1) * in.rp + (w - 1)) * (uint32_t)sizeof(double);
+        wrapb = (uint32_t)(in.rp - w) * (uint32_t)sizeof(double);
+        inb = row_at(in, -2, j0);
+        outb = row_at(out, -2, j0);
+        lc = lane - 2;
+        lo = (uint32_t)lane * (uint32_t)sizeof(double);
+        for (int m = 0; m < wraps; ++m) wrap(lc, lo);
+        sc = lc;
+        so = lo;
+    }
+    __device__ void wrap(int& c, uint32_t& o) const {
+        const bool e = c > in.nx + 1;
+        c = e ? c - w : c;
+        o = e ? o + wrapb : o;
+    }
+    __device__ void advance(int& c, uint32_t& o) const {
+        c += 64;
+        o += 64u * (uint32_t)sizeof(double);
+        for (int m = 0; m < wraps; ++m) wrap(c, o);
+    }
+    __device__ int passes() const { return (len + 63) / 64; }
+    // positions past the stream's end load its last cell
+    __device__ Cons load(int) {
+        const Cons U = ld_off(in, inb, min(lo, olast));
+        advance(lc, lo);
+        return U;
+    }
+    __device__ void store(int kk, const Cons& U2, const Cons& Fm, const Cons& Fp) {
+        const int c = sc - 2;   // the updated cell, in the lane's row
+        const bool mine = (unsigned)c < (unsigned)in.nx && 64 * kk + lane < len;
+        const Cons Unew = update(U2, dtd, Fm, Fp);
+        st_off(out, outb, mine ? so - 2u * (uint32_t)sizeof(double) : 0x80000000u, Unew);
+        if (mine && ghosts) {
+            const int s = 64 * kk + lane;
+            if (j0 == 0 && s < w) {
+                const Cons G = bc.kind[MH_EDGE_BOTTOM] == MH_BC_INFLOW ? bc.s[MH_EDGE_BOTTOM] : Unew;
+                st(out, c, -1, G);
+                st(out, c, -2, G);
+            }
+            if (j1 == in.ny && s >= len - w) {
+                const Cons G = bc.kind[MH_EDGE_TOP] == MH_BC_INFLOW ? bc.s[MH_EDGE_TOP] : Unew;
+                st(out, c, in.ny, G);
+                st(out, c, in.ny + 1, G);
+            }
+        }
+        advance(sc, so);
+    }
+};
+
+// The deferred-update x tile of x_tiled_pass over an XStream: the same
+// passes, exchanges and face solves, with the stream's loads and stores (the
+// single-grid sweeps keep their own copy of the pass, which the compiler
+// schedules 0.5 % faster than this form with the addressing passed in).
+template <int S, bool F>
+__device__ __forceinline__ bool x_stream_pass(XStream& a, double half, XLdsC& X, int lane) {
+    int it = 0;
+    const int niter = a.passes();
+    const bool l1 = lane >= 1, l2 = lane >= 2;
+    unsigned key = 0;   // running range key of the states read (fast pass)
+    Cons Fprev = {0.0, 0.0, 0.0, 0.0};
+    // reads the previous iteration's left face flux and the conserved state
+    // two lanes left (carries for lanes 0-1)
+    auto read_prev = [&](Cons& Fm, Cons& U2) {
+        double f1[4], u2[4];
+        for (int c = 0; c < 4; ++c) {
+            f1[c] = l1 ? X.M[4 + c][lane - 1] : X.cF[c];
+            u2[c] = l2 ? X.U[c][lane - 2] : X.cU[c][lane];
+        }
+        Fm = {f1[0], f1[1], f1[2], f1[3]};
+        U2 = {u2[0], u2[1], u2[2], u2[3]};
+    };
+    Cons Un = a.load(0);
+    for (int k = 0; k < niter; ++k) {
+        const Cons U0 = Un;
+        Un = a.load(k + 1);
+        const Prim W0 = c2p<F>(U0);
+        if constexpr (F) key = max(key, range_key(U0.r, U0.e, W0.p));
+        X.M[0][lane] = W0.r; X.M[1][lane] = W0.u; X.M[2][lane] = W0.v; X.M[3][lane] = W0.p;
+        wave_sync();
+        double q1[4], q2[4];
+        for (int c = 0; c < 4; ++c) {
+            q1[c] = l1 ? X.M[c][lane - 1] : X.cW[c][1];
+            q2[c] = l2 ? X.M[c][lane - 2] : X.cW[c][lane];
+        }
+        Cons Fm, U2;
+        read_prev(Fm, U2);
+        wave_sync();
+        if (lane >= 62) {
+            X.cW[0][lane - 62] = W0.r; X.cW[1][lane - 62] = W0.u; X.cW[2][lane - 62] = W0.v;
+            X.cW[3][lane - 62] = W0.p;
+            for (int c = 0; c < 4; ++c) X.cU[c][lane - 62] = X.U[c][lane];
+        }
+        X.U[0][lane] = U0.r; X.U[1][lane] = U0.mu; X.U[2][lane] = U0.mv; X.U[3][lane] = U0.e;
+        if (lane == 63) { X.cF[0] = Fprev.r; X.cF[1] = Fprev.mu; X.cF[2] = Fprev.mv; X.cF[3] = Fprev.e; }
+        if (k > 0) a.store(k - 1, U2, Fm, Fprev);
+        const Prim W1 = {q1[0], q1[1], q1[2], q1[3]};
+        const Prim W2 = {q2[0], q2[1], q2[2], q2[3]};
+        Edge L, R;
+        reconstruct<kX, F>(W2, W1, W0, half, L, R);
+        // the right edge state to the lane on the right: conserved state and
+        // u, v, p (its primitive density is the conserved one)
+        X.M[0][lane] = R.U.r; X.M[1][lane] = R.U.mu; X.M[2][lane] = R.U.mv; X.M[3][lane] = R.U.e;
+        X.M[5][lane] = R.V.u; X.M[6][lane] = R.V.v; X.M[7][lane] = R.V.p;
+        wave_sync();
+        double r1[8];
+        for (int c = 0; c < 8; ++c)
+            if (c != 4) r1[c] = l1 ? X.M[c][lane - 1] : X.cR[c];
+        wave_sync();
+        if (lane == 63) {
+            X.cR[0] = R.U.r; X.cR[1] = R.U.mu; X.cR[2] = R.U.mv; X.cR[3] = R.U.e;
+            X.cR[5] = R.V.u; X.cR[6] = R.V.v; X.cR[7] = R.V.p;
+        }
+        const Cons RU = {r1[0], r1[1], r1[2], r1[3]};
+        const Prim RV = {r1[0], r1[5], r1[6], r1[7]};
+        Fprev = riemann<S, kX, F>(RU, RV, L.U, L.V, it);
+        X.M[4][lane] = Fprev.r; X.M[5][lane] = Fprev.mu; X.M[6][lane] = Fprev.mv; X.M[7][lane] = Fprev.e;
+    }
+    wave_sync();
+    Cons Fm, U2;
+    read_prev(Fm, U2);
+    a.store(niter - 1, U2, Fm, Fprev);
+    wave_sync();   // the slice is free for the next tile of this wavefront
+    if constexpr (F) return !__any(key >= kRangeKeyEnd);
+    return true;
+}
+
+// Rows [j0, j1) of a grid as one x tile (XStream), fast pass first as in
+// x_tile; the batch kernels' x tiles of whole rows.
+template <int S>
+__device__ __forceinline__ void x_stream(const View& in, const View& out, int j0, int j1, double dtd, double half,
+                                         const Bc& bc, int ghosts, XLdsC& X, int lane) {
+    if (dt_in_fast_range(dtd)) {
+        XStream a(in, out, bc, j0, j1, ghosts, lane, dtd);
+        if (x_stream_pass<S, true>(a, half, X, lane)) return;
+    }
+    XStream a(in, out, bc, j0, j1, ghosts, lane, dtd);
+    x_stream_pass<S, false>(a, half, X, lane);
+}
+
 // Layout selector: L = 0 compact, 1 = 66-slot, 2 = compact with the update
 // deferred by one iteration.
 template <int L> struct XSlice { using type = XLdsC; };

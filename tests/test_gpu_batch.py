@@ -27,7 +27,7 @@ def oracle_instance(oracle, s, solver, steps):
 # (xseg, ychunk, mode, blocks_per_cu, xrows): persistent fused / two-launch,
 # dispatched, ragged segments and chunks, several rows per x tile
 @pytest.mark.parametrize("schedule", [(0, 256, 0, 0, 1), (0, 32, 1, 0, 1), (48, 0, 0, 1, 2), (0, 64, 2, 0, 1),
-                                      (60, 16, 2, 0, 3), (0, 64, 2, 0, 8)])
+                                      (60, 16, 2, 0, 3), (0, 64, 2, 0, 8), (0, 128, 2, 0, 1000), (0, 96, 1, 0, 7)])
 def test_batch_instances_match_oracle(hip, oracle, schedule):
     from large_sys_eq_opt_amd.ensemble import Batch
 
