@@ -67,10 +67,17 @@ __device__ __forceinline__ void vjp_store(const View& out, int i, int c, const C
     else st(out, i, c, v);
 }
 
-template <int S, int TO>
-__global__ __launch_bounds__(64) void k_vjp_y(View U, View lam, View out, double dtd, double half, int seg,
-                                              int nch, int ngroups, int fold, int lo_kind, int hi_kind) {
+// One pass of k_vjp_y over its chunk.  F (fast pass): the sweeps' fast
+// division and square root (adjoint_device.h); the range key of every state
+// read is folded into a lane maximum and one vote at the end says whether
+// the pass's results stand (returns false if not: the IEEE pass then redoes
+// the chunk, rewriting every cell the fast one stored).
+template <int S, int TO, bool F>
+__device__ __forceinline__ bool vjp_pass(const View& U, const View& lam, const View& out, double dtd, double half,
+                                         int seg, int nch, int ngroups, int fold, int lo_kind, int hi_kind,
+                                         double (&ringU)[4][4][64], double (&ringL)[4][4][64]) {
     const int lane = threadIdx.x;
+    unsigned key = 0;
     const int cg = blockIdx.x % ngroups;
     const int ch = blockIdx.x / ngroups;
     const int i = cg * 64 + lane;
@@ -81,8 +88,6 @@ __global__ __launch_bounds__(64) void k_vjp_y(View U, View lam, View out, double
     const int c1 = ch == nch - 1 ? ny + 2 : c0 + seg;
     const bool lo_tr = lo_kind == MH_BC_TRANSMISSIVE, hi_tr = hi_kind == MH_BC_TRANSMISSIVE;
     Cons acc_lo = czero(), acc_hi = czero(), held = czero();
-    __shared__ double ringU[4][4][64];
-    __shared__ double ringL[4][4][64];
     auto putU = [&](int s, const Cons& v) { ringU[s][0][lane] = v.r; ringU[s][1][lane] = v.mu; ringU[s][2][lane] = v.mv; ringU[s][3][lane] = v.e; };
     auto getU = [&](int s) { return Cons{ringU[s][0][lane], ringU[s][1][lane], ringU[s][2][lane], ringU[s][3][lane]}; };
     auto putL = [&](int s, const Cons& v) { ringL[s][0][lane] = v.r; ringL[s][1][lane] = v.mu; ringL[s][2][lane] = v.mv; ringL[s][3][lane] = v.e; };
@@ -97,31 +102,32 @@ __global__ __launch_bounds__(64) void k_vjp_y(View U, View lam, View out, double
     for (int p = c0 - 3; p <= c1 + 2; ++p) {
         const int pr = min(max(p, -2), ny + 1);
         const Cons Up = ld(U, ic, pr);
-        const Prim W0 = c2p(Up);
+        const Prim W0 = c2p<F>(Up);
+        if constexpr (F) key = max(key, range_key(Up.r, Up.e, W0.p));
         const int s0 = (p + 8) & 3;
         putU(s0, Up);
         putL(s0, (p >= 0 && p < ny) ? ld(lam, ic, p) : czero());
         // primal reconstruction of row q = p-1 (feeds the faces below and above it)
         Edge Lq, Rq;
         Lq.U = czero(); Lq.V = pzero(); Rq = Lq;
-        if (p >= c0 - 1 && p - 1 >= -1 && p - 1 <= ny) reconstruct<kY>(W2, W1, W0, half, Lq, Rq);
+        if (p >= c0 - 1 && p - 1 >= -1 && p - 1 <= ny) reconstruct<kY, F>(W2, W1, W0, half, Lq, Rq);
         // face between rows p-2 and p-1
         Cons bRf = czero(), bLf = czero();
         if (p >= c0 && p - 2 >= -1 && p - 2 <= ny - 1) {
             const Cons l1 = getL((p + 7) & 3), l2 = getL((p + 6) & 3);
             const Cons mu = {dtd * (l1.r - l2.r), dtd * (l1.mu - l2.mu), dtd * (l1.mv - l2.mv), dtd * (l1.e - l2.e)};
             Prim bRV = pzero(), bLV = pzero();
-            riemann_vjp<S, kY>(Rprev.U, Rprev.V, Lq.U, Lq.V, mu, bRf, bRV, bLf, bLV);
-            c2p_vjp(Rprev.U, bRV, bRf);
-            c2p_vjp(Lq.U, bLV, bLf);
+            riemann_vjp<S, kY, F>(Rprev.U, Rprev.V, Lq.U, Lq.V, mu, bRf, bRV, bLf, bLV);
+            c2p_vjp<F>(Rprev.U, bRV, bRf);
+            c2p_vjp<F>(Lq.U, bLV, bLf);
         }
         // reconstruction of row p-2
-        if (p >= c0 + 1 && p - 2 >= -1 && p - 2 <= ny) reconstruct_vjp<kY>(W3, W2, W1, half, bLprev, bRf, a0, a1, a2);
+        if (p >= c0 + 1 && p - 2 >= -1 && p - 2 <= ny) reconstruct_vjp<kY, F>(W3, W2, W1, half, bLprev, bRf, a0, a1, a2);
         // row c = p-3 is complete
         const int c = p - 3;
         if (c >= c0 && c < c1) {
             Cons bU = (c >= 0 && c < ny) ? getL((p + 5) & 3) : czero();
-            c2p_vjp(getU((p + 5) & 3), a0, bU);
+            c2p_vjp<F>(getU((p + 5) & 3), a0, bU);
             if (fold < 0) {
                 if (active) vjp_store<TO>(out, i, c, bU);
             } else if (c < 0) {
@@ -145,6 +151,23 @@ __global__ __launch_bounds__(64) void k_vjp_y(View U, View lam, View out, double
         bLprev = bLf;
         a0 = a1; a1 = a2; a2 = pzero();
     }
+    if constexpr (F) return !__any(key >= kRangeKeyEnd);
+    return true;
+}
+
+// The chunk's fast pass where the solver has one (HLL, HLLC, FORCE) and
+// dt/d is in the fast range, the IEEE pass where it does not or gave up.
+template <int S, int TO>
+__global__ __launch_bounds__(64) void k_vjp_y(View U, View lam, View out, double dtd, double half, int seg,
+                                              int nch, int ngroups, int fold, int lo_kind, int hi_kind) {
+    __shared__ double ringU[4][4][64];
+    __shared__ double ringL[4][4][64];
+    if constexpr (S != kExact) {
+        if (dt_in_fast_range(dtd) &&
+            vjp_pass<S, TO, true>(U, lam, out, dtd, half, seg, nch, ngroups, fold, lo_kind, hi_kind, ringU, ringL))
+            return;
+    }
+    vjp_pass<S, TO, false>(U, lam, out, dtd, half, seg, nch, ngroups, fold, lo_kind, hi_kind, ringU, ringL);
 }
 
 // dst(j, i) = src(i, j) over every cell including ghosts, rho*u <-> rho*v.

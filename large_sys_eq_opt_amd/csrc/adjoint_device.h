@@ -13,6 +13,13 @@
 // zero when a*b <= 0), fmin/fmax the operand they return (the second on
 // ties).  The time step is a constant (the adjoint of a split step at a
 // given dt).
+//
+// F (fast tiles, see the range bounds in euler_device.h): the primal values
+// recomputed here take the sweeps' fast division and square root (dv, dvf,
+// dvn, sq with F), bit-identical to the IEEE ones in a fast tile, so every
+// branch is the forward's; the derivative quotients take div_nr_nf / the
+// refined reciprocal of the same primal denominators (normal and nonzero
+// wherever the result is used), within an ulp of the IEEE quotient.
 #pragma once
 
 #include "euler_device.h"
@@ -29,8 +36,9 @@ __device__ __forceinline__ Cons czero() { return {0.0, 0.0, 0.0, 0.0}; }
 __device__ __forceinline__ Prim pzero() { return {0.0, 0.0, 0.0, 0.0}; }
 
 // V = c2p(U): u = mu/r, v = mv/r, p = (g-1)(e - r(u^2+v^2)/2).
+template <bool F = false>
 __device__ __forceinline__ void c2p_vjp(const Cons& U, const Prim& bV, Cons& bU) {
-    const double inv = 1.0 / U.r;
+    const double inv = dvn<F>(1.0, U.r);
     const double u = U.mu * inv, v = U.mv * inv;
     const double h = 0.5 * U.r, s = u * u + v * v;
     const double bkin = -kGm1 * bV.p;
@@ -44,11 +52,12 @@ __device__ __forceinline__ void c2p_vjp(const Cons& U, const Prim& bV, Cons& bU)
 }
 
 // U = p2c(V): (r, r u, r v, p/(g-1) + r(u^2+v^2)/2).
+template <bool F = false>
 __device__ __forceinline__ void p2c_vjp(const Prim& V, const Cons& bU, Prim& bV) {
     bV.r += bU.r + V.u * bU.mu + V.v * bU.mv + 0.5 * (V.u * V.u + V.v * V.v) * bU.e;
     bV.u += V.r * bU.mu + V.r * V.u * bU.e;
     bV.v += V.r * bU.mv + V.r * V.v * bU.e;
-    bV.p += bU.e / kGm1;
+    bV.p += dvn<F>(bU.e, kGm1);
 }
 
 template <int D>
@@ -69,18 +78,27 @@ __device__ __forceinline__ void pflux_vjp(const Cons& U, const Prim& V, const Co
 }
 
 // a = sqrt(g p / r)
+template <bool F = false>
 __device__ __forceinline__ void sound_vjp(const Prim& V, double a, double ba, Prim& bV) {
-    const double s = kGamma * V.p / V.r;
-    const double bs = ba / (2.0 * a);
-    bV.p += kGamma / V.r * bs;
-    bV.r -= s / V.r * bs;
+    if constexpr (F) {
+        const double rr = rcp_nr(V.r);
+        const double s = div_rr(kGamma * V.p, V.r, rr);
+        const double bs = dvn<true>(ba, 2.0 * a);
+        bV.p += div_rr(kGamma, V.r, rr) * bs;
+        bV.r -= div_rr(s, V.r, rr) * bs;
+    } else {
+        const double s = kGamma * V.p / V.r;
+        const double bs = ba / (2.0 * a);
+        bV.p += kGamma / V.r * bs;
+        bV.r -= s / V.r * bs;
+    }
 }
 
 template <int D> __device__ __forceinline__ void add_un(Prim& b, double x) { if (D == kX) b.u += x; else b.v += x; }
 template <int D> __device__ __forceinline__ void add_ut(Prim& b, double x) { if (D == kX) b.v += x; else b.u += x; }
 
 // S_L = fmin(unL - aL, unR - aR), S_R = fmax(unL + aL, unR + aR)
-template <int D>
+template <int D, bool F = false>
 __device__ __forceinline__ void speeds_vjp(const Prim& VL, const Prim& VR, double aL, double aR, double bSL,
                                            double bSR, Prim& bVL, Prim& bVR) {
     const double unL = un_of<D>(VL), unR = un_of<D>(VR);
@@ -89,24 +107,24 @@ __device__ __forceinline__ void speeds_vjp(const Prim& VL, const Prim& VR, doubl
     else { add_un<D>(bVR, bSL); baR -= bSL; }
     if ((unL + aL) > (unR + aR)) { add_un<D>(bVL, bSR); baL += bSR; }
     else { add_un<D>(bVR, bSR); baR += bSR; }
-    sound_vjp(VL, aL, baL, bVL);
-    sound_vjp(VR, aR, baR, bVR);
+    sound_vjp<F>(VL, aL, baL, bVL);
+    sound_vjp<F>(VR, aR, baR, bVR);
 }
 
 // ---------------------------------------------------------------------------
 // HLL (head/riemann.hpp:87-129)
 // ---------------------------------------------------------------------------
-template <int D>
+template <int D, bool F = false>
 __device__ void hll_vjp(const Cons& UL, const Prim& VL, const Cons& UR, const Prim& VR, const Cons& bF, Cons& bUL,
                         Prim& bVL, Cons& bUR, Prim& bVR) {
     const Cons FL = pflux<D>(UL, VL), FR = pflux<D>(UR, VR);
-    if (!phys_riemann(VL) || !phys_riemann(VR)) {
+    if (!phys_riemann<F>(VL) || !phys_riemann<F>(VR)) {
         const Cons h = {0.5 * bF.r, 0.5 * bF.mu, 0.5 * bF.mv, 0.5 * bF.e};
         pflux_vjp<D>(UL, VL, h, bUL, bVL);
         pflux_vjp<D>(UR, VR, h, bUR, bVR);
         return;
     }
-    const double aL = sound(VL), aR = sound(VR);
+    const double aL = sound<F>(VL), aR = sound<F>(VR);
     const double unL = un_of<D>(VL), unR = un_of<D>(VR);
     const double SL = lmin(unL - aL, unR - aR);
     const double SR = lmax(unL + aL, unR + aR);
@@ -120,14 +138,16 @@ __device__ void hll_vjp(const Cons& UL, const Prim& VL, const Cons& UR, const Pr
         return;
     }
     const double sls = SL * SR;
+    const double rd = F ? rcp_nr(denom) : 0.0;   // denom >= 1e-14 in magnitude here
+    auto qd = [&](double x) { return F ? div_rr(x, denom, rd) : x / denom; };
     const double fl[4] = {FL.r, FL.mu, FL.mv, FL.e}, fr[4] = {FR.r, FR.mu, FR.mv, FR.e};
     const double ul[4] = {UL.r, UL.mu, UL.mv, UL.e}, ur[4] = {UR.r, UR.mu, UR.mv, UR.e};
     const double bf[4] = {bF.r, bF.mu, bF.mv, bF.e};
     double bSL = 0.0, bSR = 0.0, bden = 0.0, bfl[4], bfr[4], bul[4], bur[4];
     for (int c = 0; c < 4; ++c) {
         const double N = SR * fl[c] - SL * fr[c] + sls * (ur[c] - ul[c]);
-        const double bN = bf[c] / denom;
-        bden -= (N / denom) * bf[c] / denom;
+        const double bN = qd(bf[c]);
+        bden -= qd(qd(N) * bf[c]);
         bSR += fl[c] * bN + SL * (ur[c] - ul[c]) * bN;
         bSL += -fr[c] * bN + SR * (ur[c] - ul[c]) * bN;
         bfl[c] = SR * bN;
@@ -141,17 +161,17 @@ __device__ void hll_vjp(const Cons& UL, const Prim& VL, const Cons& UR, const Pr
     acc(bUR, bur[0], bur[1], bur[2], bur[3]);
     pflux_vjp<D>(UL, VL, Cons{bfl[0], bfl[1], bfl[2], bfl[3]}, bUL, bVL);
     pflux_vjp<D>(UR, VR, Cons{bfr[0], bfr[1], bfr[2], bfr[3]}, bUR, bVR);
-    speeds_vjp<D>(VL, VR, aL, aR, bSL, bSR, bVL, bVR);
+    speeds_vjp<D, F>(VL, VR, aL, aR, bSL, bSR, bVL, bVR);
 }
 
 // ---------------------------------------------------------------------------
 // HLLC (head/riemann.hpp:135-271)
 // ---------------------------------------------------------------------------
-template <int D>
+template <int D, bool F = false>
 __device__ void hllc_vjp(const Cons& UL, const Prim& VL, const Cons& UR, const Prim& VR, const Cons& bF, Cons& bUL,
                          Prim& bVL, Cons& bUR, Prim& bVR) {
-    if (!phys_riemann(VL) || !phys_riemann(VR)) { hll_vjp<D>(UL, VL, UR, VR, bF, bUL, bVL, bUR, bVR); return; }
-    const double aL = sound(VL), aR = sound(VR);
+    if (!phys_riemann<F>(VL) || !phys_riemann<F>(VR)) { hll_vjp<D, F>(UL, VL, UR, VR, bF, bUL, bVL, bUR, bVR); return; }
+    const double aL = sound<F>(VL), aR = sound<F>(VR);
     const double unL = un_of<D>(VL), unR = un_of<D>(VR);
     const double pL = VL.p, pR = VR.p, rhoL = VL.r, rhoR = VR.r;
     const double SL = lmin(unL - aL, unR - aR);
@@ -160,9 +180,9 @@ __device__ void hllc_vjp(const Cons& UL, const Prim& VL, const Cons& UR, const P
     if (SR <= 0.0) { pflux_vjp<D>(UR, VR, bF, bUR, bVR); return; }
     const double numerator = pR - pL + rhoL * unL * (SL - unL) - rhoR * unR * (SR - unR);
     const double denominator = rhoL * (SL - unL) - rhoR * (SR - unR);
-    if (fabs(denominator) < 1.0e-14) { hll_vjp<D>(UL, VL, UR, VR, bF, bUL, bVL, bUR, bVR); return; }
-    const double Ss = numerator / denominator;
-    if (!finite(Ss)) { hll_vjp<D>(UL, VL, UR, VR, bF, bUL, bVL, bUR, bVR); return; }
+    if (fabs(denominator) < 1.0e-14) { hll_vjp<D, F>(UL, VL, UR, VR, bF, bUL, bVL, bUR, bVR); return; }
+    const double Ss = dv<F>(numerator, denominator);
+    if (!finite(Ss)) { hll_vjp<D, F>(UL, VL, UR, VR, bF, bUL, bVL, bUR, bVR); return; }
     const bool left = Ss >= 0.0;
     const Cons& U = left ? UL : UR;
     const Prim& V = left ? VL : VR;
@@ -176,11 +196,11 @@ __device__ void hllc_vjp(const Cons& UL, const Prim& VL, const Cons& UR, const P
     double rs = 0.0, Epr = 0.0, Q = 0.0, P = 0.0, A = 0.0, Bq = 0.0, inner = 0.0, Es = 0.0;
     if (star) {
         Q = rho * (S - un);
-        rs = Q / dS;
+        rs = dvf<F>(Q, dS);
         star = finite(rs) && rs > 0.0;
         if (star) {
-            Epr = U.e / rho;
-            P = p / Q;
+            Epr = dvn<F>(U.e, rho);
+            P = dvn<F>(p, Q);
             A = Ss - un;
             Bq = Ss + P;
             inner = Epr + A * Bq;
@@ -217,12 +237,12 @@ __device__ void hllc_vjp(const Cons& UL, const Prim& VL, const Cons& UR, const P
         double bun = 0.0, brho = 0.0, bp = 0.0, be = 0.0, bQ = 0.0;
         bSs += bA + bBq;
         bun -= bA;
-        bp += bBq / Q;
-        bQ -= P / Q * bBq;
-        be += bEpr / rho;
-        brho -= Epr / rho * bEpr;
-        bQ += brs / dS;
-        const double bdS = -rs / dS * brs;
+        bp += dvn<F>(bBq, Q);
+        bQ -= dvn<F>(P, Q) * bBq;
+        be += dvn<F>(bEpr, rho);
+        brho -= dvn<F>(Epr, rho) * bEpr;
+        bQ += dvn<F>(brs, dS);
+        const double bdS = dvn<F>(-rs, dS) * brs;
         brho += (S - un) * bQ;
         const double bD = rho * bQ;
         bS += bD;
@@ -236,8 +256,8 @@ __device__ void hllc_vjp(const Cons& UL, const Prim& VL, const Cons& UR, const P
         bU.e += be;
     }
     // S* = num / den
-    const double bnum = bSs / denominator;
-    const double bden = -Ss / denominator * bSs;
+    const double bnum = dvn<F>(bSs, denominator);
+    const double bden = dvn<F>(-Ss, denominator) * bSs;
     double bSL = left ? bS : 0.0, bSR = left ? 0.0 : bS;
     bVR.p += bnum;
     bVL.p -= bnum;
@@ -247,7 +267,7 @@ __device__ void hllc_vjp(const Cons& UL, const Prim& VL, const Cons& UR, const P
     bVR.r += -unR * (SR - unR) * bnum - (SR - unR) * bden;
     add_un<D>(bVR, -rhoR * (SR - 2.0 * unR) * bnum + rhoR * bden);
     bSR += -rhoR * unR * bnum - rhoR * bden;
-    speeds_vjp<D>(VL, VR, aL, aR, bSL, bSR, bVL, bVR);
+    speeds_vjp<D, F>(VL, VR, aL, aR, bSL, bSR, bVL, bVR);
 }
 
 // ---------------------------------------------------------------------------
@@ -415,11 +435,11 @@ __device__ void exact_vjp(const Cons& UL, const Prim& VL, const Cons& UR, const 
 // (FR - FL), alpha = fmax(|unL| + aL, |unR| + aR); |x| takes slope -1 only
 // for x < 0.
 // ---------------------------------------------------------------------------
-template <int D>
+template <int D, bool F = false>
 __device__ void force_vjp(const Cons& UL, const Prim& VL, const Cons& UR, const Prim& VR, const Cons& bF, Cons& bUL,
                           Prim& bVL, Cons& bUR, Prim& bVR) {
-    if (!phys_riemann(VL) || !phys_riemann(VR)) { hll_vjp<D>(UL, VL, UR, VR, bF, bUL, bVL, bUR, bVR); return; }
-    const double aL = sound(VL), aR = sound(VR);
+    if (!phys_riemann<F>(VL) || !phys_riemann<F>(VR)) { hll_vjp<D, F>(UL, VL, UR, VR, bF, bUL, bVL, bUR, bVR); return; }
+    const double aL = sound<F>(VL), aR = sound<F>(VR);
     const double unL = un_of<D>(VL), unR = un_of<D>(VR);
     const double alpha = lmax(fabs(unL) + aL, fabs(unR) + aR);
     const Cons FL = pflux<D>(UL, VL), FR = pflux<D>(UR, VR);
@@ -429,14 +449,14 @@ __device__ void force_vjp(const Cons& UL, const Prim& VL, const Cons& UR, const 
         pflux_vjp<D>(UR, VR, h, bUR, bVR);
         return;
     }
-    const double ha = 0.5 * alpha, hi = 0.5 * (1.0 / alpha);
+    const double ha = 0.5 * alpha, hi = 0.5 * dvn<F>(1.0, alpha);
     const double fl[4] = {FL.r, FL.mu, FL.mv, FL.e}, fr[4] = {FR.r, FR.mu, FR.mv, FR.e};
     const double ul[4] = {UL.r, UL.mu, UL.mv, UL.e}, ur[4] = {UR.r, UR.mu, UR.mv, UR.e};
     const double bf[4] = {bF.r, bF.mu, bF.mv, bF.e};
     const Cons Uri = {0.5 * (ul[0] + ur[0]) - hi * (fr[0] - fl[0]), 0.5 * (ul[1] + ur[1]) - hi * (fr[1] - fl[1]),
                       0.5 * (ul[2] + ur[2]) - hi * (fr[2] - fl[2]), 0.5 * (ul[3] + ur[3]) - hi * (fr[3] - fl[3])};
-    const Prim Vri = c2p(Uri);
-    const bool ri = phys_riemann(Vri);
+    const Prim Vri = c2p<F>(Uri);
+    const bool ri = phys_riemann<F>(Vri);
     double blf[4], bri[4];
     for (int c = 0; c < 4; ++c) { blf[c] = ri ? 0.5 * bf[c] : bf[c]; bri[c] = ri ? 0.5 * bf[c] : 0.0; }
     double bfl[4] = {0.0, 0.0, 0.0, 0.0}, bfr[4] = {0.0, 0.0, 0.0, 0.0}, bul[4], bur[4];
@@ -445,7 +465,7 @@ __device__ void force_vjp(const Cons& UL, const Prim& VL, const Cons& UR, const 
         Cons bUri = czero();
         Prim bVri = pzero();
         pflux_vjp<D>(Uri, Vri, Cons{bri[0], bri[1], bri[2], bri[3]}, bUri, bVri);
-        c2p_vjp(Uri, bVri, bUri);
+        c2p_vjp<F>(Uri, bVri, bUri);
         const double bu[4] = {bUri.r, bUri.mu, bUri.mv, bUri.e};
         double bhi = 0.0;
         for (int c = 0; c < 4; ++c) {
@@ -453,7 +473,7 @@ __device__ void force_vjp(const Cons& UL, const Prim& VL, const Cons& UR, const 
             bfr[c] -= hi * bu[c];
             bfl[c] += hi * bu[c];
         }
-        balpha += bhi * (-0.5 / (alpha * alpha));
+        balpha += bhi * dvn<F>(-0.5, alpha * alpha);
         for (int c = 0; c < 4; ++c) { bul[c] = 0.5 * bu[c]; bur[c] = 0.5 * bu[c]; }
     } else {
         for (int c = 0; c < 4; ++c) { bul[c] = 0.0; bur[c] = 0.0; }
@@ -479,17 +499,18 @@ __device__ void force_vjp(const Cons& UL, const Prim& VL, const Cons& UR, const 
         add_un<D>(bVR, unR < 0.0 ? -balpha : balpha);
         baR += balpha;
     }
-    sound_vjp(VL, aL, baL, bVL);
-    sound_vjp(VR, aR, baR, bVR);
+    sound_vjp<F>(VL, aL, baL, bVL);
+    sound_vjp<F>(VR, aR, baR, bVR);
 }
 
-template <int S, int D>
+// F: HLL, HLLC and FORCE only (the exact solver keeps the IEEE forms).
+template <int S, int D, bool F = false>
 __device__ __forceinline__ void riemann_vjp(const Cons& UL, const Prim& VL, const Cons& UR, const Prim& VR,
                                             const Cons& bF, Cons& bUL, Prim& bVL, Cons& bUR, Prim& bVR) {
-    if (S == kHLLC) hllc_vjp<D>(UL, VL, UR, VR, bF, bUL, bVL, bUR, bVR);
+    if (S == kHLLC) hllc_vjp<D, F>(UL, VL, UR, VR, bF, bUL, bVL, bUR, bVR);
     else if (S == kExact) exact_vjp<D>(UL, VL, UR, VR, bF, bUL, bVL, bUR, bVR);
-    else if (S == kForce) force_vjp<D>(UL, VL, UR, VR, bF, bUL, bVL, bUR, bVR);
-    else hll_vjp<D>(UL, VL, UR, VR, bF, bUL, bVL, bUR, bVR);
+    else if (S == kForce) force_vjp<D, F>(UL, VL, UR, VR, bF, bUL, bVL, bUR, bVR);
+    else hll_vjp<D, F>(UL, VL, UR, VR, bF, bUL, bVL, bUR, bVR);
 }
 
 // ---------------------------------------------------------------------------
@@ -497,7 +518,7 @@ __device__ __forceinline__ void riemann_vjp(const Cons& UL, const Prim& VL, cons
 // edge states (their conserved parts, primitive parts already folded in) to
 // the primitive states of the three cells.
 // ---------------------------------------------------------------------------
-template <int D>
+template <int D, bool F = false>
 __device__ void reconstruct_vjp(const Prim& Wm, const Prim& Wc, const Prim& Wp, double half, const Cons& bLU,
                                 const Cons& bRU, Prim& bWm, Prim& bWc, Prim& bWp) {
     const double am[4] = {Wc.r - Wm.r, Wc.u - Wm.u, Wc.v - Wm.v, Wc.p - Wm.p};
@@ -512,16 +533,16 @@ __device__ void reconstruct_vjp(const Prim& Wm, const Prim& Wc, const Prim& Wp, 
     const Prim hs = {0.5 * sl[0], 0.5 * sl[1], 0.5 * sl[2], 0.5 * sl[3]};
     Prim WL = {Wc.r - hs.r, Wc.u - hs.u, Wc.v - hs.v, Wc.p - hs.p};
     Prim WR = {Wc.r + hs.r, Wc.u + hs.u, Wc.v + hs.v, Wc.p + hs.p};
-    const bool okL = phys_recon(WL), okR = phys_recon(WR);
+    const bool okL = phys_recon<F>(WL), okR = phys_recon<F>(WR);
     if (!okL) WL = Wc;
     if (!okR) WR = Wc;
-    const Cons UL = p2c(WL), UR = p2c(WR);
-    const Prim VL = c2p(UL), VR = c2p(UR);
+    const Cons UL = p2c<F>(WL), UR = p2c<F>(WR);
+    const Prim VL = c2p<F>(UL), VR = c2p<F>(UR);
     const Cons FL = pflux<D>(UL, VL), FR = pflux<D>(UR, VR);
     const Cons h = {half * (FR.r - FL.r), half * (FR.mu - FL.mu), half * (FR.mv - FL.mv), half * (FR.e - FL.e)};
     const Cons cL = {UL.r - h.r, UL.mu - h.mu, UL.mv - h.mv, UL.e - h.e};
     const Cons cR = {UR.r - h.r, UR.mu - h.mu, UR.mv - h.mv, UR.e - h.e};
-    const bool eL = phys_recon(c2p(cL)), eR = phys_recon(c2p(cR));
+    const bool eL = phys_recon<F>(c2p<F>(cL)), eR = phys_recon<F>(c2p<F>(cR));
     // backward
     Cons bUL = czero(), bUR = czero(), bh = czero();
     acc(bUL, bLU.r, bLU.mu, bLU.mv, bLU.e);
@@ -533,11 +554,11 @@ __device__ void reconstruct_vjp(const Prim& Wm, const Prim& Wc, const Prim& Wp, 
     Prim bVL = pzero(), bVR = pzero();
     pflux_vjp<D>(UL, VL, bFL, bUL, bVL);
     pflux_vjp<D>(UR, VR, bFR, bUR, bVR);
-    c2p_vjp(UL, bVL, bUL);
-    c2p_vjp(UR, bVR, bUR);
+    c2p_vjp<F>(UL, bVL, bUL);
+    c2p_vjp<F>(UR, bVR, bUR);
     Prim bWL = pzero(), bWR = pzero();
-    p2c_vjp(WL, bUL, bWL);
-    p2c_vjp(WR, bUR, bWR);
+    p2c_vjp<F>(WL, bUL, bWL);
+    p2c_vjp<F>(WR, bUR, bWR);
     acc(bWc, bWL.r, bWL.u, bWL.v, bWL.p);
     acc(bWc, bWR.r, bWR.u, bWR.v, bWR.p);
     double bhs[4] = {0.0, 0.0, 0.0, 0.0};

@@ -142,6 +142,30 @@ def test_sweep_vjp_dot_products(hip, oracle, solver, shape):
 
 
 @pytest.mark.parametrize("solver", [HLL, HLLC, FORCE])
+def test_sweep_vjp_dot_products_outside_fast_range(hip, oracle, solver):
+    """A band of rows scaled to densities near 1e-7 (below the fast range's
+    2^-20; velocities and sound speeds unchanged, so dt is too): the product's
+    chunks and column groups that read the band take the IEEE pass, the
+    others the fast one, and both meet the same 1e-12 bar."""
+    nx, ny = 150, 66
+    U = smooth_state(oracle, nx, ny, seed=15)
+    U[:30] *= 1e-7
+    dt = _dt(oracle, U, nx, ny)
+    rng = np.random.default_rng(8)
+    a = _adjoint(hip, nx, ny, solver)
+    for d in (0, 1):
+        lam = _lam(rng, nx, ny)
+        got = _sweep_vjp(hip, a, U, dt, d, lam)
+        for _ in range(3):
+            v = rng.standard_normal(U.shape)
+            v[:30] *= 1e-7
+            _, dout = oracle.sweep_tangent(U, v, 1.0 / nx, 1.0 / ny, dt, d, solver)
+            lhs = np.sum(interior(lam) * interior(dout))
+            rhs = np.sum(got * v)
+            assert abs(lhs - rhs) <= 1e-12 * np.sum(np.abs(got * v)), (d, lhs, rhs)
+
+
+@pytest.mark.parametrize("solver", [HLL, HLLC, FORCE])
 def test_forward_backward_checkpointed(hip, oracle, solver):
     from large_sys_eq_opt_amd.solver import Boundary
 
