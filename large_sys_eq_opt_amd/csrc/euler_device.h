@@ -423,11 +423,43 @@ __device__ __forceinline__ Cons exact_sample(const Prim& VK, double aK, double p
     return cons_nt(rho, un, ut_of<D>(VK), p, D);
 }
 
+// The exact solver's result for a face whose two primitive states are equal
+// (numerically; a uniform region), with 1e-12 <= p < 1e300 and rho, a below
+// 1e300 (so the PVRS guess's sums stay finite): what exact()
+// computes there, without the calls to pow, which all take a ratio of 1.
+// The PVRS guess is p (0.5 (p + p) = p, the velocity difference is zero);
+// the first Newton step evaluates both pressure functions at p = p_K
+// (rarefaction branch, pow(1, e) = 1): f = +0, f' = 2 / (rho a), so
+// p* = p - 0 / f' = p, the step is zero and the iteration stops;
+// u* = 0.5 ((u_n + u_n + 0) - 0) (the sum as exact() forms it: +0 for a
+// zero velocity); the sampled state on the side u* picks is (rho, u*, p)
+// on every wave-speed branch (the head and tail speeds coincide, so no
+// fan).  Then the same conserved state, conversion, check and flux.
+template <int D>
+__device__ __forceinline__ Cons exact_uniform(const Cons& UL, const Prim& VL, const Cons& UR, const Prim& VR,
+                                              double a, int& iters) {
+    iters = 1;
+    const double fd1 = (1.0 / (VL.r * a)) * 1.0;
+    if (fabs(fd1 + fd1) < 1.0e-14) return hll<D>(UL, VL, UR, VR);
+    const double unL = un_of<D>(VL), unR = un_of<D>(VR);
+    const double u_star = 0.5 * (unL + unR + 0.0 - 0.0);
+    if (!finite(u_star)) return hll<D>(UL, VL, UR, VR);
+    const Prim& VK = u_star >= 0.0 ? VL : VR;
+    const Cons Us = cons_nt(VK.r, u_star, ut_of<D>(VK), VK.p, D);
+    const Prim Vs = c2p(Us);
+    if (!phys_riemann(Vs)) return hll<D>(UL, VL, UR, VR);
+    return pflux<D>(Us, Vs);
+}
+
 template <int D>
 __device__ __forceinline__ Cons exact(const Cons& UL, const Prim& VL, const Cons& UR, const Prim& VR, int& iters) {
     iters = 0;
     if (!phys_riemann(VL) || !phys_riemann(VR)) return hll<D>(UL, VL, UR, VR);
     const double aL = sound(VL), aR = sound(VR);
+    // a wavefront whose faces all join equal states takes the closed form
+    const bool uniform = VL.r == VR.r && VL.u == VR.u && VL.v == VR.v && VL.p == VR.p && VL.p >= 1.0e-12 &&
+                         VL.p < 1.0e300 && VL.r < 1.0e300 && aL < 1.0e300;
+    if (__all(uniform)) return exact_uniform<D>(UL, VL, UR, VR, aL, iters);
     double p_star = 0.0, u_star = 0.0;
     const bool ok = exact_star<D>(VL, VR, aL, aR, p_star, u_star, iters);
     if (!ok || !finite(p_star) || !finite(u_star) || p_star <= 0.0) return hll<D>(UL, VL, UR, VR);

@@ -81,6 +81,30 @@ def test_riemann_flux_parity(hip, oracle, solver, direction):
 
 
 @pytest.mark.parametrize("direction", [0, 1])
+def test_exact_uniform_faces_bitwise(hip, oracle, direction):
+    """Faces whose two states are equal: a wavefront of only such faces takes
+    the exact solver's closed form (no pow: every pressure ratio is 1), one
+    with a single other face the full Newton path.  Both give the oracle's
+    bits, which take the full path through pow(1, e) = 1 (no rounding of pow
+    enters)."""
+    rng = np.random.default_rng(77 + direction)
+    V = random_states(rng, 256)
+    V[:8, 1 + direction] = [0.0, -0.0, 5.0, -5.0, 0.3, -0.3, 1e-300, -1e-300]   # still, supersonic, tiny
+    V[8:16, 2 - direction] = -0.0
+    S = oracle.prim_to_cons(V)
+    UL = np.concatenate([S, S])
+    UR = np.concatenate([S, S])
+    UR[256 + 64 * np.arange(4) + 17] = oracle.prim_to_cons(random_states(rng, 4))   # one other face per wavefront
+    Fo = oracle.riemann_flux(UL, UR, direction, 2)
+    Fg, it = gpu_flux(hip, UL, UR, direction, 2)
+    assert same_bits(Fg[:256], Fo[:256])
+    uni = np.ones(512, bool)
+    uni[256 + 64 * np.arange(4) + 17] = False
+    assert same_bits(Fg[256:][uni[256:]], Fg[:256][uni[256:]])
+    assert (it[:256] == 1).all()
+
+
+@pytest.mark.parametrize("direction", [0, 1])
 def test_reconstruction_parity(hip, oracle, direction):
     rng = np.random.default_rng(7 + direction)
     V = [random_states(rng, 4000) for _ in range(3)]
