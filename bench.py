@@ -342,6 +342,25 @@ def copy_peak(nbytes: int = 4 << 30, reps: int = 10):
     return 2.0 * nbytes / dt
 
 
+def smooth_state(n):
+    """An (n+4)^2 ghosted AoS state with no two equal neighbours: smooth
+    density, velocity and pressure waves across the unit square (rho in
+    [0.7, 1.3], p in [0.6, 1.4], |u|, |v| <= 0.3)."""
+    import numpy as np
+
+    y, x = (np.mgrid[0:n + 4, 0:n + 4] + 0.5) / (n + 4)
+    rho = 1.0 + 0.3 * np.sin(2 * np.pi * x + 0.3) * np.cos(2 * np.pi * y + 1.1)
+    u = 0.3 * np.sin(2 * np.pi * (y + 0.37 * x) + 0.7)
+    v = 0.2 * np.cos(2 * np.pi * (x - 0.41 * y) + 2.0)
+    p = 1.0 + 0.4 * np.cos(2 * np.pi * (x + 0.83 * y) + 0.4)
+    U = np.empty((n + 4, n + 4, 4))
+    U[..., 0] = rho
+    U[..., 1] = rho * u
+    U[..., 2] = rho * v
+    U[..., 3] = p / 0.4 + 0.5 * rho * (u * u + v * v)
+    return U
+
+
 def main():
     if "--leg" in sys.argv and "cpu" in sys.argv:
         _set_omp_env()
@@ -356,6 +375,9 @@ def main():
     ap.add_argument("--leg", choices=["batched", "adjoint", "snapshot", "cpu"], default=None,
                     help="run only this extra leg (batched: against --plain-rate Gcell/s)")
     ap.add_argument("--plain-rate", type=float, default=32.0)
+    ap.add_argument("--state", choices=["case", "smooth"], default="case",
+                    help="case: the cylindrical blast's initial state; smooth: a smooth state with no two equal "
+                         "neighbours (no uniform region), on the same grid")
     ap.add_argument("--copy-only", type=int, default=0,
                     help="only run the copy kernel on buffers of this many MiB (counter calibration)")
     args = ap.parse_args()
@@ -393,6 +415,8 @@ def main():
     kind, params = cases.case_params("cyl_blast")
     U = Grid2DGPU.from_config(cfg)
     U.init_state(kind, params.array())
+    if args.state == "smooth":
+        U.upload_from_aos(smooth_state(n))
     st = Stepper(U, solver, cfg.cfl, 0.0, cases.case_boundary("cyl_blast"),
                  log_capacity=args.steps + args.warmup + 8)
     if args.warmup:
@@ -415,7 +439,7 @@ def main():
     achieved = BYTES_PER_SWEEP * cells / (kern[dom]["avg_ms"] * 1e-3)
     traffic = None
     tfile = os.path.join(ROOT, "profiles", f"traffic_{dom}_{args.solver.lower()}_cyl_blast_{n}.json")
-    if os.path.exists(tfile):
+    if args.state == "case" and os.path.exists(tfile):
         with open(tfile) as f:
             traffic = json.load(f).get("bytes_per_launch")
     out = {
@@ -431,7 +455,8 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": f"cyl_blast {n}x{n} {args.solver.upper()} FP64 MUSCL-Hancock, x-then-y split, "
+        "config": {"workload": (f"cyl_blast {n}x{n}" if args.state == "case" else f"smooth non-uniform state {n}x{n}")
+                               + f" {args.solver.upper()} FP64 MUSCL-Hancock, x-then-y split, "
                                f"device-resident dt (clock + x sweep + y sweep per step)",
                    "nx": cfg.nx, "ny": cfg.ny, "solver": args.solver, "cfl": cfg.cfl, "precision": "fp64"},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
