@@ -133,7 +133,12 @@ int mh_sweep(mh_grid in, mh_grid out, double dt, int dir, int solver, mh_stream 
  *      clamps it to t_end - t, advances, and accumulates t.  --------------- */
 /* Creates a stepper over state U (owned by the caller) with its own scratch
  * grid.  t_end <= 0: no clamp, steps run unconditionally.  bc NULL =
- * transmissive.  Fills the ghosts of U and reduces its max signal speed. */
+ * transmissive.  Fills the ghosts of U and reduces its max signal speed.
+ * With the launch shape's `fused` set (the default) and the fused step
+ * applicable (nx a multiple of 64; HLL, HLLC or FORCE), each step is one
+ * pass and the state alternates between U and the scratch grid: it is in U
+ * again after mh_stepper_clock, mh_stepper_finish or mh_stepper_destroy,
+ * which is when U may be read. */
 int mh_stepper_create(mh_grid U, int solver, double cfl, double t_end, const mh_bc* bc, int dt_log_capacity,
                       mh_stepper* out, mh_stream stream);
 int mh_stepper_destroy(mh_stepper s);
@@ -143,7 +148,8 @@ int mh_stepper_advance(mh_stepper s, int nsteps, mh_stream stream);
  * capacity entries; may be NULL).  Returns MH_ERR_WAVESPEED if a step met a
  * non-positive max signal speed. */
 int mh_stepper_clock(mh_stepper s, double* t, int* steps, double* dt_log, mh_stream stream);
-/* Re-fills the ghosts of the state (after the last step). */
+/* Re-fills the ghosts of the state (after the last step); synchronises
+ * first if steps were enqueued since the last mh_stepper_clock. */
 int mh_stepper_finish(mh_stepper s, mh_stream stream);
 /* Holds the loop after the first step with t >= t_stop (0: no stop) -- the
  * reference writes a snapshot there (scripts/gpu/main_gpu.cu:272-298); steps
@@ -152,8 +158,10 @@ int mh_stepper_finish(mh_stepper s, mh_stream stream);
 int mh_stepper_set_stop(mh_stepper s, double t_stop, mh_stream stream);
 /* mh_stepper_advance with a HIP event pair around every launch; synchronises.
  * ms_total: elapsed time of the n steps; ms_kernels[3]: average duration per
- * step of the clock kernel, the x sweep and the y sweep.  (No reference
- * counterpart: the reference times whole runs externally.) */
+ * step of the clock kernel, the x sweep and the y sweep -- or, for a stepper
+ * on the fused step, of the clock kernel, the fused step (with its redo
+ * pass) and 0.  (No reference counterpart: the reference times whole runs
+ * externally.) */
 int mh_stepper_advance_profiled(mh_stepper s, int nsteps, mh_stream stream, double* ms_total, double* ms_kernels);
 /* Launch shape of the sweeps (process-wide; chosen by the autotuner).  No
  * reference counterpart: src/gpu/solver_gpu.cu:830-831 hard-codes 16x16
@@ -170,6 +178,10 @@ typedef struct {
                       y: at least 4 rows) and, while still below it, one
                       tile per workgroup; 0 keeps the shape as set
                       (default 8192) */
+    int fused;    /* 1 (default): steppers created from now on run each step
+                     as one fused x-then-y pass where it applies (nx a
+                     multiple of 64; HLL, HLLC, FORCE), with seg_y rows per
+                     chunk; 0: the x sweep and the y sweep */
 } mh_launch_shape;
 int mh_set_launch_shape(const mh_launch_shape* shape);
 int mh_get_launch_shape(mh_launch_shape* shape);

@@ -50,6 +50,7 @@ int launch_sweep_host(const mh_grid_s* in, const mh_grid_s* out, int dir, int so
                       int ghosts, hipStream_t st);
 int max_speed_host(const mh_grid_s* g, double* smax, hipStream_t st);
 int stepper_start(mh_stepper s, double cfl, hipStream_t st);
+void stepper_unfuse(mh_stepper s);
 
 // Output rows -2..ny+1 are cut into nch chunks of seg rows, the last one
 // running to row ny+1 (seg..2*seg-1 rows), so that rows -2..0 share the
@@ -399,6 +400,7 @@ int mh_adjoint_create(const mh_grid_desc* desc, int solver, const mh_bc* bc, int
         if (!rc) rc = check_hip(hipEventCreateWithFlags(&a->vjp_done[b], hipEventDisableTiming), "hipEventCreate");
     }
     if (!rc) rc = mh_stepper_create(a->work, solver, 1.0, 0.0, bc, std::max(max_steps, 1), &a->stepper, nullptr);
+    if (!rc) stepper_unfuse(a->stepper);
     // Cells no kernel writes (the ghost rows of the ping-pong grids) read
     // as zero.
     for (mh_grid_s* g : {a->bTt, a->A0, a->A1})

@@ -35,6 +35,7 @@
 #include "euler_device.h"
 #include "grid.h"
 #include "sweep_tiles.h"
+#include "fused_tile.h"
 
 namespace mh {
 
@@ -59,10 +60,13 @@ int check_hip(hipError_t e, const char* what) {
 // Device clock of the device-resident time loop.  t_stop > 0 holds the
 // loop after the first step with t >= t_stop (a snapshot point): the clock
 // sets done and stopped, and mh_stepper_set_stop releases it.
+// With pp set (a stepper on the fused step), committed steps alternate the
+// state between the stepper's grid (cur = 0) and its second grid (cur = 1).
 struct Clock {
     double t, dt, t_end, cfl, h, t_stop;
     unsigned long long smax[2];   // bit patterns of non-negative doubles
     int step, slot, pending, done, status, log_cap, stopped;
+    int pp, cur;
 };
 
 // dt source of a sweep: a host value or the device clock.
@@ -205,6 +209,108 @@ __global__ __launch_bounds__(64 * WPB) void k_sweep_x(View in, View out, SweepAr
 }
 
 // ---------------------------------------------------------------------------
+// fused split step (fused_tile.h): one workgroup of two wavefronts per
+// 64-column strip and chunk of rows; the x wavefront feeds the y wavefront
+// through LDS.  A tile whose states leave the fast-path range (or a step
+// whose dt/dx or dt/dy is outside it) goes on the output grid's redo list,
+// and k_step_fused_redo runs its IEEE pass; the fast pass's speed maximum of
+// such a tile is dropped.
+// ---------------------------------------------------------------------------
+struct FusedArgs {
+    DtSrc dts;
+    double dx, dy;
+    int seg;           // rows per chunk
+    int ngroups;       // 64-column strips
+    int ntiles;
+    int ghosts;
+    Bc bc;
+    unsigned long long* speed;
+    int clock_speed;
+    int* redo;
+    int pp;            // views g0/g1 alternate as input/output with the clock's cur
+};
+
+__device__ __forceinline__ unsigned long long* speed_slot(const FusedArgs& a) {
+    return a.clock_speed ? &a.dts.clock->smax[a.dts.clock->slot ^ 1] : a.speed;
+}
+
+template <int S, bool F>
+__device__ __forceinline__ void fused_tile(const View& in, const View& out, const FusedArgs& a, int tile, double dt,
+                                           PairLds& X, int w, int lane, bool& ok, double& m) {
+    const double dtdx = dt / a.dx, dtdy = dt / a.dy;
+    const int cg = tile % a.ngroups;
+    const int j0 = (tile / a.ngroups) * a.seg;
+    const int j1 = min(j0 + a.seg, in.ny);
+    const int want = (a.speed || a.clock_speed) ? 1 : 0;
+    if (w == 0) {
+        pair_x<S, F>(in, cg * 64, j0, j1, dtdx, 0.5 * dtdx, a.bc, X, lane);
+        ok = true;
+    } else {
+        ok = pair_y<S, F>(out, cg * 64, j0, j1, dtdy, 0.5 * dtdy, a.bc, a.ghosts, want, X, lane, m);
+        ok = ok && X.xok != 0u;
+    }
+}
+
+template <int S>
+__global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_step_fused(View g0, View g1,
+                                                                                             FusedArgs a) {
+    double dt;
+    if (!dt_of(a.dts, dt)) return;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const int tile = blockIdx.x;
+    if (tile >= a.ntiles) return;
+    const bool swap = a.pp && a.dts.clock->cur;
+    const View in = swap ? g1 : g0, out = swap ? g0 : g1;
+    __shared__ PairLds X;
+    if (!(dt_in_fast_range(dt / a.dx) && dt_in_fast_range(dt / a.dy))) {
+        if (w == 1 && lane == 0) a.redo[2 + atomicAdd(a.redo, 1)] = tile;
+        return;
+    }
+    bool ok = true;
+    double m = 0.0;
+    fused_tile<S, true>(in, out, a, tile, dt, X, w, lane, ok, m);
+    if (w == 1) {
+        if (!ok) {
+            if (lane == 0) a.redo[2 + atomicAdd(a.redo, 1)] = tile;
+        } else if (a.speed || a.clock_speed) {
+            m = wave_max(m);
+            if (lane == 0) atomic_max_pos(speed_slot(a), m);
+        }
+    }
+}
+
+// IEEE pass over the tiles k_step_fused put on the redo list (both
+// wavefronts of a workgroup walk the same tiles, with a barrier between
+// tiles so the ring is free); the last workgroup to finish empties the list.
+template <int S>
+__global__ __launch_bounds__(128) void k_step_fused_redo(View g0, View g1, FusedArgs a) {
+    __shared__ PairLds X;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const int n = __hip_atomic_load(a.redo, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    double dt;
+    if (n > 0 && dt_of(a.dts, dt)) {
+        const bool swap = a.pp && a.dts.clock->cur;
+        const View in = swap ? g1 : g0, out = swap ? g0 : g1;
+        for (int q = blockIdx.x; q < n; q += gridDim.x) {
+            const int tile = a.redo[2 + q];
+            bool ok = true;
+            double m = 0.0;
+            fused_tile<S, false>(in, out, a, tile, dt, X, w, lane, ok, m);
+            if (w == 1 && (a.speed || a.clock_speed)) {
+                m = wave_max(m);
+                if (lane == 0) atomic_max_pos(speed_slot(a), m);
+            }
+            pair_barrier();
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0 && atomicAdd(a.redo + 1, 1) == (int)gridDim.x - 1) {
+        a.redo[0] = 0;
+        a.redo[1] = 0;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // first-order unsplit Godunov step (src/cpu/solver_cpu.cpp:315-369)
 // ---------------------------------------------------------------------------
 template <int S>
@@ -258,6 +364,7 @@ __global__ void k_clock(Clock* c, double* dt_log, int commit_only) {
         c->step += 1;
         c->pending = 0;
         c->slot ^= 1;
+        c->cur ^= c->pp;
     }
     if (commit_only || c->done) return;
     if (c->t_end > 0.0 && !(c->t < c->t_end)) {
@@ -284,8 +391,10 @@ __global__ void k_clock(Clock* c, double* dt_log, int commit_only) {
 }
 
 // Starts a clock at t = 0 with both speed slots clear.
-__global__ void k_clock_init(Clock* c, double t_end, double cfl, double h, int log_cap) {
+__global__ void k_clock_init(Clock* c, double t_end, double cfl, double h, int log_cap, int pp) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    c->pp = pp;
+    c->cur = 0;
     c->t = 0.0;
     c->dt = 0.0;
     c->t_end = t_end;
@@ -301,6 +410,11 @@ __global__ void k_clock_init(Clock* c, double t_end, double cfl, double h, int l
     c->status = 0;
     c->log_cap = log_cap;
     c->stopped = 0;
+}
+
+// The host has swapped the stepper's two grids: the state is in the first.
+__global__ void k_clock_cur0(Clock* c) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) c->cur = 0;
 }
 
 // Moves the clock's stop time; releases a clock held at the previous one.
@@ -475,7 +589,7 @@ int valid_bc(const mh_bc* b) {
 // The shape is process-wide: mh_set_launch_shape replaces it under a lock
 // and every launch reads one consistent copy (shape()), so a host thread
 // that sets it while another launches races on nothing.
-static mh_launch_shape g_shape = {1020, 128, 4, 2, 1, 4, 8192};
+static mh_launch_shape g_shape = {1020, 128, 4, 2, 1, 4, 8192, 1};
 static std::mutex g_shape_lock;
 
 mh_launch_shape shape() {
@@ -587,6 +701,63 @@ int launch_sweep(const mh_grid_s* in, const mh_grid_s* out, int dir, int solver,
     return check_hip(hipGetLastError(), "sweep launch");
 }
 
+// The fused step applies to grids whose rows split into whole 64-column
+// strips, with the solvers that have the fast division path.
+bool fused_applies(const mh_grid_s* g, int solver) { return solver != kExact && g->d.nx % 64 == 0; }
+
+// Rows per chunk of the fused step: the configured y chunk, shortened (to
+// no fewer than 16 rows: each chunk recomputes 4 warm-up rows) while the
+// grid gives fewer pairs than min_tiles / 2 (the pairs hold two wavefronts
+// each; by default 4096, two rounds of the eight a CU holds).
+int seg_fused_for(const mh_grid_s* g, const mh_launch_shape& sh) {
+    const int ngroups = g->d.nx / 64, ny = g->d.ny;
+    int seg = sh.seg_y > 0 ? sh.seg_y : ny;
+    const int want = sh.min_tiles / 2;
+    if ((int64_t)ngroups * ((ny + seg - 1) / seg) < want && seg > 16) {
+        const int chunks = (want + ngroups - 1) / ngroups;
+        seg = std::min(seg, std::max(16, (ny + chunks - 1) / chunks));
+    }
+    return seg;
+}
+
+int fused_tiles(const mh_grid_s* g) {
+    const int seg = seg_fused_for(g, shape());
+    return (g->d.nx / 64) * ((g->d.ny + seg - 1) / seg);
+}
+
+template <int S>
+void go_fused(hipStream_t st, const View& v0, const View& v1, const FusedArgs& a) {
+    hipLaunchKernelGGL(k_step_fused<S>, dim3((unsigned)a.ntiles), dim3(128), 0, st, v0, v1, a);
+    hipLaunchKernelGGL(k_step_fused_redo<S>, dim3(1024), dim3(128), 0, st, v0, v1, a);
+}
+
+// One fused step g0 -> g1 (pp = 0), or between g0 and g1 as the clock's cur
+// says (pp = 1).  `redo` is a redo list with room for every tile.
+int launch_step_fused(const mh_grid_s* g0, const mh_grid_s* g1, int solver, DtSrc dts, const Bc& bc, int ghosts,
+                      unsigned long long* speed, int clock_speed, int pp, int* redo, hipStream_t st) {
+    FusedArgs a;
+    a.dts = dts;
+    a.dx = g0->dx;
+    a.dy = g0->dy;
+    a.seg = seg_fused_for(g0, shape());
+    a.ngroups = g0->d.nx / 64;
+    a.ntiles = a.ngroups * ((g0->d.ny + a.seg - 1) / a.seg);
+    a.ghosts = ghosts;
+    a.bc = bc;
+    a.speed = speed;
+    a.clock_speed = clock_speed;
+    a.redo = redo;
+    a.pp = pp;
+    const View v0 = g0->view(), v1 = g1->view();
+    switch (solver) {
+        case kHLL: go_fused<kHLL>(st, v0, v1, a); break;
+        case kHLLC: go_fused<kHLLC>(st, v0, v1, a); break;
+        case kForce: go_fused<kForce>(st, v0, v1, a); break;
+        default: return fail(MH_ERR_INVALID, "fused step: solver without the fused path");
+    }
+    return check_hip(hipGetLastError(), "fused step launch");
+}
+
 int launch_fill(const mh_grid_s* g, const Bc& bc, hipStream_t st) {
     const int n = 4 * g->d.ny + 4 * (g->d.nx + 4);
     hipLaunchKernelGGL(k_fill_ghosts, dim3((n + 255) / 256), dim3(256), 0, st, g->view(), bc);
@@ -634,6 +805,10 @@ struct mh_stepper_s {
     Clock* clock;      // device
     double* dt_log;    // device
     int log_cap;
+    // fused: steps run as k_step_fused, alternating the state between U and
+    // tmp (the clock's cur); dirty: steps were enqueued since the host last
+    // moved the state back into U (stepper_settle)
+    int fused, dirty;
 };
 
 namespace mh {
@@ -643,12 +818,53 @@ namespace mh {
 int stepper_start(mh_stepper s, double cfl, hipStream_t st) {
     s->cfl = cfl;
     hipLaunchKernelGGL(k_clock_init, dim3(1), dim3(64), 0, st, s->clock, s->t_end, cfl,
-                       std::min(s->U->dx, s->U->dy), s->log_cap);
+                       std::min(s->U->dx, s->U->dy), s->log_cap, s->fused);
+    s->dirty = 0;
     int rc = check_hip(hipGetLastError(), "clock launch");
     if (!rc) rc = launch_fill(s->U, s->bc, st);
     if (!rc) rc = launch_max_speed(s->U, &s->clock->smax[0], st);
     return rc;
 }
+
+// Steps of a stepper: the fused step where it applies, else the x sweep
+// U -> tmp and the y sweep tmp -> U.
+int stepper_step(mh_stepper s, hipStream_t st) {
+    const DtSrc dts{s->clock, 0.0};
+    s->dirty = 1;
+    if (s->fused) return launch_step_fused(s->U, s->tmp, s->solver, dts, s->bc, 1, nullptr, 1, 1, s->tmp->redo, st);
+    int rc = launch_sweep(s->U, s->tmp, kX, s->solver, dts, s->bc, 1, nullptr, 0, st);
+    // The y sweep reduces the new state's max signal speed into the
+    // clock's inactive slot, which k_clock has just cleared.
+    if (!rc) rc = launch_sweep(s->tmp, s->U, kY, s->solver, dts, s->bc, 1, nullptr, 1, st);
+    return rc;
+}
+
+// After a fused run whose committed steps left the state in tmp (clock
+// cur = 1, read by the caller from a synchronised copy c), swaps the two
+// grids' storage so that the stepper's grid holds it again.
+int stepper_settle(mh_stepper s, const Clock& c, hipStream_t st) {
+    s->dirty = 0;
+    if (!s->fused || !c.cur) return MH_OK;
+    std::swap(s->U->base, s->tmp->base);
+    hipLaunchKernelGGL(k_clock_cur0, dim3(1), dim3(64), 0, st, s->clock);
+    return check_hip(hipGetLastError(), "clock launch");
+}
+
+// Reads the clock (synchronising `st`) and settles the state.
+int stepper_sync(mh_stepper s, hipStream_t st, Clock* out) {
+    Clock c;
+    hipLaunchKernelGGL(k_clock, dim3(1), dim3(64), 0, st, s->clock, s->dt_log, 1);
+    int rc = check_hip(hipGetLastError(), "clock launch");
+    if (!rc) rc = check_hip(hipMemcpyAsync(&c, s->clock, sizeof(c), hipMemcpyDeviceToHost, st), "hipMemcpyAsync");
+    if (!rc) rc = check_hip(hipStreamSynchronize(st), "hipStreamSynchronize");
+    if (!rc) rc = stepper_settle(s, c, st);
+    if (!rc && out) *out = c;
+    return rc;
+}
+
+// The adjoint's forward run copies checkpoints out of the stepper's grid
+// between steps, so its stepper keeps the two sweeps.
+void stepper_unfuse(mh_stepper s) { s->fused = 0; }
 }  // namespace mh
 
 extern "C" {
@@ -892,6 +1108,8 @@ int mh_stepper_create(mh_grid U, int solver, double cfl, double t_end, const mh_
     s->log_cap = dt_log_capacity > 0 ? dt_log_capacity : 0;
     int rc = mh_grid_create(&U->d, &s->tmp);
     if (rc) { delete s; return rc; }
+    s->fused = shape().fused && fused_applies(U, solver) && fused_tiles(U) <= s->tmp->redo_cap;
+    s->dirty = 0;
     rc = check_hip(hipMalloc((void**)&s->clock, sizeof(Clock)), "hipMalloc(clock)");
     s->dt_log = nullptr;
     if (!rc && s->log_cap) rc = check_hip(hipMalloc((void**)&s->dt_log, sizeof(double) * s->log_cap), "hipMalloc(dt log)");
@@ -904,6 +1122,8 @@ int mh_stepper_create(mh_grid U, int solver, double cfl, double t_end, const mh_
 
 int mh_stepper_destroy(mh_stepper s) {
     if (!s) return MH_OK;
+    // the state goes back into the caller's grid before tmp is freed
+    if (s->dirty && s->clock && s->tmp) (void)stepper_sync(s, nullptr, nullptr);
     if (s->tmp) mh_grid_destroy(s->tmp);
     if (s->clock) (void)hipFree(s->clock);
     if (s->dt_log) (void)hipFree(s->dt_log);
@@ -914,14 +1134,10 @@ int mh_stepper_destroy(mh_stepper s) {
 int mh_stepper_advance(mh_stepper s, int nsteps, mh_stream stream) {
     if (!s || nsteps < 0) return fail(MH_ERR_INVALID, "mh_stepper_advance: bad argument");
     hipStream_t st = (hipStream_t)stream;
-    const DtSrc dts{s->clock, 0.0};
     for (int k = 0; k < nsteps; ++k) {
         hipLaunchKernelGGL(k_clock, dim3(1), dim3(64), 0, st, s->clock, s->dt_log, 0);
         int rc = check_hip(hipGetLastError(), "clock launch");
-        // The y sweep reduces the new state's max signal speed into the
-        // clock's inactive slot, which k_clock has just cleared.
-        if (!rc) rc = launch_sweep(s->U, s->tmp, kX, s->solver, dts, s->bc, 1, nullptr, 0, st);
-        if (!rc) rc = launch_sweep(s->tmp, s->U, kY, s->solver, dts, s->bc, 1, nullptr, 1, st);
+        if (!rc) rc = stepper_step(s, st);
         if (rc) return rc;
     }
     return MH_OK;
@@ -930,12 +1146,10 @@ int mh_stepper_advance(mh_stepper s, int nsteps, mh_stream stream) {
 int mh_stepper_clock(mh_stepper s, double* t, int* steps, double* dt_log, mh_stream stream) {
     if (!s) return fail(MH_ERR_INVALID, "mh_stepper_clock: null stepper");
     hipStream_t st = (hipStream_t)stream;
-    hipLaunchKernelGGL(k_clock, dim3(1), dim3(64), 0, st, s->clock, s->dt_log, 1);
     Clock c;
-    int rc = check_hip(hipGetLastError(), "clock launch");
-    if (!rc) rc = check_hip(hipMemcpyAsync(&c, s->clock, sizeof(c), hipMemcpyDeviceToHost, st), "hipMemcpyAsync");
-    // c.step sizes the log copy: wait for the clock first
-    if (!rc) rc = check_hip(hipStreamSynchronize(st), "hipStreamSynchronize");
+    // c.step sizes the log copy: the clock is read (and the state settled
+    // into the stepper's grid) first
+    int rc = stepper_sync(s, st, &c);
     if (!rc && dt_log && s->log_cap && c.step > 0) {
         rc = check_hip(hipMemcpyAsync(dt_log, s->dt_log, sizeof(double) * std::min(s->log_cap, c.step),
                                       hipMemcpyDeviceToHost, st),
@@ -958,6 +1172,10 @@ int mh_stepper_set_stop(mh_stepper s, double t_stop, mh_stream stream) {
 
 int mh_stepper_finish(mh_stepper s, mh_stream stream) {
     if (!s) return fail(MH_ERR_INVALID, "mh_stepper_finish: null stepper");
+    if (s->dirty) {
+        const int rc = stepper_sync(s, (hipStream_t)stream, nullptr);
+        if (rc) return rc;
+    }
     return launch_fill(s->U, s->bc, (hipStream_t)stream);
 }
 
@@ -1174,7 +1392,8 @@ int mh_stepper_advance_profiled(mh_stepper s, int nsteps, mh_stream stream, doub
     if (!s || nsteps < 1) return fail(MH_ERR_INVALID, "mh_stepper_advance_profiled: bad argument");
     hipStream_t st = (hipStream_t)stream;
     const DtSrc dts{s->clock, 0.0};
-    // events: [k][0..3] = before clock, before x, before y, after y
+    // events: [k][0..3] = before clock, before x, before y, after y (fused:
+    // before the step, after it, after it)
     std::vector<hipEvent_t> ev((size_t)nsteps * 4);
     int rc = MH_OK;
     for (auto& e : ev)
@@ -1185,9 +1404,15 @@ int mh_stepper_advance_profiled(mh_stepper s, int nsteps, mh_stream stream, doub
         hipLaunchKernelGGL(k_clock, dim3(1), dim3(64), 0, st, s->clock, s->dt_log, 0);
         if (!rc) rc = check_hip(hipGetLastError(), "clock launch");
         if (!rc) rc = check_hip(hipEventRecord(e[1], st), "hipEventRecord");
-        if (!rc) rc = launch_sweep(s->U, s->tmp, kX, s->solver, dts, s->bc, 1, nullptr, 0, st);
-        if (!rc) rc = check_hip(hipEventRecord(e[2], st), "hipEventRecord");
-        if (!rc) rc = launch_sweep(s->tmp, s->U, kY, s->solver, dts, s->bc, 1, nullptr, 1, st);
+        if (s->fused) {
+            if (!rc) rc = stepper_step(s, st);
+            if (!rc) rc = check_hip(hipEventRecord(e[2], st), "hipEventRecord");
+        } else {
+            s->dirty = 1;
+            if (!rc) rc = launch_sweep(s->U, s->tmp, kX, s->solver, dts, s->bc, 1, nullptr, 0, st);
+            if (!rc) rc = check_hip(hipEventRecord(e[2], st), "hipEventRecord");
+            if (!rc) rc = launch_sweep(s->tmp, s->U, kY, s->solver, dts, s->bc, 1, nullptr, 1, st);
+        }
         if (!rc) rc = check_hip(hipEventRecord(e[3], st), "hipEventRecord");
     }
     if (!rc) rc = check_hip(hipStreamSynchronize(st), "hipStreamSynchronize");
@@ -1206,6 +1431,7 @@ int mh_stepper_advance_profiled(mh_stepper s, int nsteps, mh_stream stream, doub
         if (e) (void)hipEventDestroy(e);
     if (rc) return rc;
     if (ms_total) *ms_total = tot;
+    if (s->fused) sum[2] = 0.0;   // the step was one launch (and its redo pass)
     if (ms_kernels)
         for (int q = 0; q < 3; ++q) ms_kernels[q] = sum[q] / nsteps;
     return MH_OK;
@@ -1216,6 +1442,7 @@ int mh_set_launch_shape(const mh_launch_shape* s) {
     if (s->seg_x < 0 || s->seg_y < 0) return fail(MH_ERR_INVALID, "mh_set_launch_shape: negative segment");
     if (s->min_tiles < 0) return fail(MH_ERR_INVALID, "mh_set_launch_shape: negative min_tiles");
     if (s->y_waves != 1 && s->y_waves != 4) return fail(MH_ERR_INVALID, "mh_set_launch_shape: y_waves must be 1 or 4");
+    if (s->fused != 0 && s->fused != 1) return fail(MH_ERR_INVALID, "mh_set_launch_shape: fused must be 0 or 1");
     if (s->x_layout < 0 || s->x_layout > 2) return fail(MH_ERR_INVALID, "mh_set_launch_shape: x_layout must be 0, 1 or 2");
     if ((s->x_wpb != 1 && s->x_wpb != 4) || (s->y_wpb != 1 && s->y_wpb != 4))
         return fail(MH_ERR_INVALID, "mh_set_launch_shape: wavefronts per workgroup must be 1 or 4");

@@ -1,0 +1,161 @@
+"""The fused split step (one pass per step: an x wavefront feeding a y
+wavefront through LDS) gives the x sweep followed by the y sweep bit for bit.
+
+Device-loop runs on grids whose width is a multiple of 64 (the fused path's
+domain) against the oracle: ragged heights and chunk sizes (partial last
+chunks, chunks shorter than the 32-row window), states on both sides of the
+fast-division range (tiles redone by the IEEE pass), tiny momenta (the HLL /
+HLLC numerators that take the IEEE division), a state so cold that every
+tile is redone, and inflow boundaries; the stepper's state alternates
+between its two grids and has to come back into the caller's grid.  At the
+benchmark size the fused steps are compared with the two-sweep steps.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from conftest import interior, same_bits
+from test_gpu_fast_range import scaled_field, tiny_momentum_field
+
+pytestmark = pytest.mark.gpu
+
+BITWISE_SOLVERS = [0, 1, 3]
+
+
+def _run(hip, cfg, solver, U0, steps, shape, bc=None):
+    from large_sys_eq_opt_amd import Grid2DGPU, Stepper
+    from large_sys_eq_opt_amd.solver import get_launch_shape, set_launch_shape
+
+    saved = get_launch_shape()
+    try:
+        set_launch_shape(**shape)
+        g = Grid2DGPU.from_config(cfg)
+        g.upload_from_aos(U0)
+        st = Stepper(g, solver, cfg.cfl, 0.0, bc, log_capacity=steps)
+        st.advance(steps)
+        st.finish()
+        _, n, dts = st.clock()
+        st.close()
+        out = g.download_to_aos()
+        g.close()
+    finally:
+        set_launch_shape(**saved)
+    return n, dts, out
+
+
+def _check(o, n, dts, out, steps):
+    assert n == steps
+    assert same_bits(dts, o["dt"])
+    assert same_bits(interior(out), interior(o["U"]))
+
+
+@pytest.mark.parametrize("solver", BITWISE_SOLVERS)
+@pytest.mark.parametrize("nx,ny,seg", [(64, 40, 16), (192, 150, 37), (320, 257, 128), (128, 300, 0)])
+def test_fused_mixed_range_bitwise(hip, oracle, solver, nx, ny, seg):
+    """Eleven steps (an odd count: the state ends in the stepper's second
+    grid and is swapped back) of the two-blob field, part of it outside the
+    fast range."""
+    from large_sys_eq_opt_amd import cases
+
+    steps = 11
+    cfg = cases.CaseConfig(nx, ny, 2, 0.0, 1.0, 0.0, 1.0, 0.8, 0.0)
+    U0 = scaled_field(oracle, nx, ny, 7)
+    o = oracle.run(U0, cfg.dx, cfg.dy, solver, cfg.cfl, max_steps=steps)
+    n, dts, out = _run(hip, cfg, solver, U0, steps, dict(fused=1, seg_y=seg, min_tiles=0))
+    _check(o, n, dts, out, steps)
+
+
+@pytest.mark.parametrize("solver", BITWISE_SOLVERS)
+def test_fused_blast_bitwise(hip, oracle, solver):
+    """The cylindrical blast on 256x256 ([0,2]^2), 30 steps, default shape."""
+    from large_sys_eq_opt_amd import cases
+
+    steps = 30
+    cfg = cases.CaseConfig(256, 256, 2, 0.0, 2.0, 0.0, 2.0, 0.8, 0.0)
+    kind, params = cases.case_params("cyl_blast")
+    U0 = oracle.init(kind, params.array(), 256, 256, cfg.bounds)
+    o = oracle.run(U0, cfg.dx, cfg.dy, solver, cfg.cfl, max_steps=steps)
+    n, dts, out = _run(hip, cfg, solver, U0, steps, dict(fused=1))
+    _check(o, n, dts, out, steps)
+
+
+@pytest.mark.parametrize("solver", BITWISE_SOLVERS)
+def test_fused_tiny_momenta_bitwise(hip, oracle, solver):
+    from large_sys_eq_opt_amd import cases
+
+    nx, ny, steps = 320, 260, 12
+    cfg = cases.CaseConfig(nx, ny, 2, 0.0, 1.0, 0.0, 1.0, 0.8, 0.0)
+    U0 = tiny_momentum_field(oracle, nx, ny, 5)
+    o = oracle.run(U0, cfg.dx, cfg.dy, solver, cfg.cfl, max_steps=steps)
+    n, dts, out = _run(hip, cfg, solver, U0, steps, dict(fused=1))
+    _check(o, n, dts, out, steps)
+
+
+def test_fused_every_tile_redone(hip, oracle):
+    """A cold, nearly still gas (p = 2e-13, below the fast range and the
+    reconstruction's floor; |u|, |v| ~ 1e-8): dt/dx exceeds 2^20, so every
+    tile of every step goes to the redo list without a fast pass."""
+    from large_sys_eq_opt_amd import cases
+
+    nx, ny, steps = 192, 130, 5
+    cfg = cases.CaseConfig(nx, ny, 2, 0.0, 1.0, 0.0, 1.0, 0.8, 0.0)
+    rng = np.random.default_rng(3)
+    V = np.empty((ny + 4, nx + 4, 4))
+    V[..., 0] = 1.0 + 0.2 * rng.random((ny + 4, nx + 4))
+    V[..., 1] = 1e-8 * rng.standard_normal((ny + 4, nx + 4))
+    V[..., 2] = 1e-8 * rng.standard_normal((ny + 4, nx + 4))
+    V[..., 3] = 2e-13 * (1.0 + 0.1 * rng.random((ny + 4, nx + 4)))
+    U0 = oracle.prim_to_cons(V.reshape(-1, 4)).reshape(ny + 4, nx + 4, 4)
+    o = oracle.run(U0, cfg.dx, cfg.dy, 1, cfg.cfl, max_steps=steps)
+    assert o["dt"][0] / cfg.dx > 2.0 ** 20
+    n, dts, out = _run(hip, cfg, 1, U0, steps, dict(fused=1))
+    _check(o, n, dts, out, steps)
+
+
+@pytest.mark.parametrize("solver", BITWISE_SOLVERS)
+def test_fused_inflow_bitwise(hip, oracle, solver):
+    """The shock-bubble case (inflow at the left) on 256x128, 25 steps:
+    the intermediate state's ghost rows and the inflow columns."""
+    from large_sys_eq_opt_amd import cases
+
+    steps, case = 25, "shock_bubble_nd"
+    cfg = cases.get_case_config(case)
+    kind, params = cases.case_params(case)
+    nx, ny = 256, 128
+    c = cases.CaseConfig(nx, ny, 2, cfg.x_min, cfg.x_max, cfg.y_min, cfg.y_max, cfg.cfl, 0.0)
+    U0 = oracle.init(kind, params.array(), nx, ny, c.bounds)
+    bc = cases.case_boundary(case)
+    o = oracle.run(U0, c.dx, c.dy, solver, c.cfl, max_steps=steps, inflow_left=bc.left_inflow_state)
+    n, dts, out = _run(hip, c, solver, U0, steps, dict(fused=1, seg_y=48, min_tiles=0), bc=bc)
+    _check(o, n, dts, out, steps)
+
+
+def test_fused_equals_sweeps_8192(hip):
+    """Three HLLC steps of the 8192^2 blast: fused and two-sweep steppers
+    give the same dt sequence and state bit for bit."""
+    from large_sys_eq_opt_amd import Grid2DGPU, Stepper, cases
+    from large_sys_eq_opt_amd.solver import get_launch_shape, set_launch_shape
+
+    n, steps = 8192, 3
+    cfg = cases.get_n_case_config("cyl_blast", n // 256)
+    kind, params = cases.case_params("cyl_blast")
+    res = []
+    saved = get_launch_shape()
+    try:
+        for fused in (1, 0):
+            set_launch_shape(fused=fused)
+            g = Grid2DGPU.from_config(cfg)
+            g.init_state(kind, params.array())
+            st = Stepper(g, 1, cfg.cfl, 0.0, cases.case_boundary("cyl_blast"), log_capacity=steps)
+            st.advance(steps)
+            st.finish()
+            _, k, dts = st.clock()
+            st.close()
+            assert k == steps
+            res.append((dts, g.download_to_aos()[::7, ::5]))
+            g.close()
+    finally:
+        set_launch_shape(**saved)
+    assert same_bits(res[0][0], res[1][0])
+    assert same_bits(res[0][1], res[1][1])
