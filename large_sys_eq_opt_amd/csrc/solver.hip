@@ -35,7 +35,7 @@
 #include "euler_device.h"
 #include "grid.h"
 #include "sweep_tiles.h"
-#include "fused_tile.h"
+#include "clock.h"
 
 namespace mh {
 
@@ -57,43 +57,6 @@ int check_hip(hipError_t e, const char* what) {
 // ---------------------------------------------------------------------------
 // device clock and sweep kernels
 // ---------------------------------------------------------------------------
-// Device clock of the device-resident time loop.  t_stop > 0 holds the
-// loop after the first step with t >= t_stop (a snapshot point): the clock
-// sets done and stopped, and mh_stepper_set_stop releases it.
-// With pp set (a stepper on the fused step), committed steps alternate the
-// state between the stepper's grid (cur = 0) and its second grid (cur = 1).
-struct Clock {
-    double t, dt, t_end, cfl, h, t_stop;
-    unsigned long long smax[2];   // bit patterns of non-negative doubles
-    int step, slot, pending, done, status, log_cap, stopped;
-    int pp, cur;
-};
-
-// dt source of a sweep: a host value or the device clock.
-struct DtSrc {
-    Clock* clock;
-    double dt;
-};
-
-__device__ __forceinline__ bool dt_of(const DtSrc& s, double& dt) {
-    if (s.clock) {
-        if (s.clock->done) return false;
-        dt = s.clock->dt;
-    } else {
-        dt = s.dt;
-    }
-    return true;
-}
-
-// Max of positive doubles through their bit patterns (monotone for m > 0).
-// The slot only grows while a kernel runs, so a read that already shows a
-// value >= m makes the atomic unnecessary; a stale smaller read only costs
-// the atomic.
-__device__ __forceinline__ void atomic_max_pos(unsigned long long* slot, double m) {
-    const unsigned long long b = (unsigned long long)__double_as_longlong(m);
-    if (m > 0.0 && __hip_atomic_load(slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < b) atomicMax(slot, b);
-}
-
 struct SweepArgs {
     DtSrc dts;
     double d;          // dx or dy
@@ -209,108 +172,6 @@ __global__ __launch_bounds__(64 * WPB) void k_sweep_x(View in, View out, SweepAr
 }
 
 // ---------------------------------------------------------------------------
-// fused split step (fused_tile.h): one workgroup of two wavefronts per
-// 64-column strip and chunk of rows; the x wavefront feeds the y wavefront
-// through LDS.  A tile whose states leave the fast-path range (or a step
-// whose dt/dx or dt/dy is outside it) goes on the output grid's redo list,
-// and k_step_fused_redo runs its IEEE pass; the fast pass's speed maximum of
-// such a tile is dropped.
-// ---------------------------------------------------------------------------
-struct FusedArgs {
-    DtSrc dts;
-    double dx, dy;
-    int seg;           // rows per chunk
-    int ngroups;       // 64-column strips
-    int ntiles;
-    int ghosts;
-    Bc bc;
-    unsigned long long* speed;
-    int clock_speed;
-    int* redo;
-    int pp;            // views g0/g1 alternate as input/output with the clock's cur
-};
-
-__device__ __forceinline__ unsigned long long* speed_slot(const FusedArgs& a) {
-    return a.clock_speed ? &a.dts.clock->smax[a.dts.clock->slot ^ 1] : a.speed;
-}
-
-template <int S, bool F>
-__device__ __forceinline__ void fused_tile(const View& in, const View& out, const FusedArgs& a, int tile, double dt,
-                                           PairLds& X, int w, int lane, bool& ok, double& m) {
-    const double dtdx = dt / a.dx, dtdy = dt / a.dy;
-    const int cg = tile % a.ngroups;
-    const int j0 = (tile / a.ngroups) * a.seg;
-    const int j1 = min(j0 + a.seg, in.ny);
-    const int want = (a.speed || a.clock_speed) ? 1 : 0;
-    if (w == 0) {
-        pair_x<S, F>(in, cg * 64, j0, j1, dtdx, 0.5 * dtdx, a.bc, X, lane);
-        ok = true;
-    } else {
-        ok = pair_y<S, F>(out, cg * 64, j0, j1, dtdy, 0.5 * dtdy, a.bc, a.ghosts, want, X, lane, m);
-        ok = ok && X.xok != 0u;
-    }
-}
-
-template <int S>
-__global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_step_fused(View g0, View g1,
-                                                                                             FusedArgs a) {
-    double dt;
-    if (!dt_of(a.dts, dt)) return;
-    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
-    const int tile = blockIdx.x;
-    if (tile >= a.ntiles) return;
-    const bool swap = a.pp && a.dts.clock->cur;
-    const View in = swap ? g1 : g0, out = swap ? g0 : g1;
-    __shared__ PairLds X;
-    if (!(dt_in_fast_range(dt / a.dx) && dt_in_fast_range(dt / a.dy))) {
-        if (w == 1 && lane == 0) a.redo[2 + atomicAdd(a.redo, 1)] = tile;
-        return;
-    }
-    bool ok = true;
-    double m = 0.0;
-    fused_tile<S, true>(in, out, a, tile, dt, X, w, lane, ok, m);
-    if (w == 1) {
-        if (!ok) {
-            if (lane == 0) a.redo[2 + atomicAdd(a.redo, 1)] = tile;
-        } else if (a.speed || a.clock_speed) {
-            m = wave_max(m);
-            if (lane == 0) atomic_max_pos(speed_slot(a), m);
-        }
-    }
-}
-
-// IEEE pass over the tiles k_step_fused put on the redo list (both
-// wavefronts of a workgroup walk the same tiles, with a barrier between
-// tiles so the ring is free); the last workgroup to finish empties the list.
-template <int S>
-__global__ __launch_bounds__(128) void k_step_fused_redo(View g0, View g1, FusedArgs a) {
-    __shared__ PairLds X;
-    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
-    const int n = __hip_atomic_load(a.redo, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    double dt;
-    if (n > 0 && dt_of(a.dts, dt)) {
-        const bool swap = a.pp && a.dts.clock->cur;
-        const View in = swap ? g1 : g0, out = swap ? g0 : g1;
-        for (int q = blockIdx.x; q < n; q += gridDim.x) {
-            const int tile = a.redo[2 + q];
-            bool ok = true;
-            double m = 0.0;
-            fused_tile<S, false>(in, out, a, tile, dt, X, w, lane, ok, m);
-            if (w == 1 && (a.speed || a.clock_speed)) {
-                m = wave_max(m);
-                if (lane == 0) atomic_max_pos(speed_slot(a), m);
-            }
-            pair_barrier();
-        }
-    }
-    __syncthreads();
-    if (threadIdx.x == 0 && atomicAdd(a.redo + 1, 1) == (int)gridDim.x - 1) {
-        a.redo[0] = 0;
-        a.redo[1] = 0;
-    }
-}
-
-// ---------------------------------------------------------------------------
 // first-order unsplit Godunov step (src/cpu/solver_cpu.cpp:315-369)
 // ---------------------------------------------------------------------------
 template <int S>
@@ -357,8 +218,10 @@ __global__ void k_zero_u64(unsigned long long* p, int n) {
 // device clock: commit the pending step, then (unless commit_only) form the
 // next dt = cfl*h/smax clamped to t_end - t (scripts/cpu/main_cpu.cpp:221-228).
 // ---------------------------------------------------------------------------
-__global__ void k_clock(Clock* c, double* dt_log, int commit_only) {
+// `redo` (a fused stepper's redo list, or null) is emptied for the step.
+__global__ void k_clock(Clock* c, double* dt_log, int commit_only, int* redo) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    if (redo) redo[0] = 0;
     if (c->pending) {
         c->t += c->dt;
         c->step += 1;
@@ -701,62 +564,11 @@ int launch_sweep(const mh_grid_s* in, const mh_grid_s* out, int dir, int solver,
     return check_hip(hipGetLastError(), "sweep launch");
 }
 
-// The fused step applies to grids whose rows split into whole 64-column
-// strips, with the solvers that have the fast division path.
-bool fused_applies(const mh_grid_s* g, int solver) { return solver != kExact && g->d.nx % 64 == 0; }
-
-// Rows per chunk of the fused step: the configured y chunk, shortened (to
-// no fewer than 16 rows: each chunk recomputes 4 warm-up rows) while the
-// grid gives fewer pairs than min_tiles / 2 (the pairs hold two wavefronts
-// each; by default 4096, two rounds of the eight a CU holds).
-int seg_fused_for(const mh_grid_s* g, const mh_launch_shape& sh) {
-    const int ngroups = g->d.nx / 64, ny = g->d.ny;
-    int seg = sh.seg_y > 0 ? sh.seg_y : ny;
-    const int want = sh.min_tiles / 2;
-    if ((int64_t)ngroups * ((ny + seg - 1) / seg) < want && seg > 16) {
-        const int chunks = (want + ngroups - 1) / ngroups;
-        seg = std::min(seg, std::max(16, (ny + chunks - 1) / chunks));
-    }
-    return seg;
-}
-
-int fused_tiles(const mh_grid_s* g) {
-    const int seg = seg_fused_for(g, shape());
-    return (g->d.nx / 64) * ((g->d.ny + seg - 1) / seg);
-}
-
-template <int S>
-void go_fused(hipStream_t st, const View& v0, const View& v1, const FusedArgs& a) {
-    hipLaunchKernelGGL(k_step_fused<S>, dim3((unsigned)a.ntiles), dim3(128), 0, st, v0, v1, a);
-    hipLaunchKernelGGL(k_step_fused_redo<S>, dim3(1024), dim3(128), 0, st, v0, v1, a);
-}
-
-// One fused step g0 -> g1 (pp = 0), or between g0 and g1 as the clock's cur
-// says (pp = 1).  `redo` is a redo list with room for every tile.
+// fused.hip
+bool fused_applies(const mh_grid_s* g, int solver);
+int fused_tiles(const mh_grid_s* g);
 int launch_step_fused(const mh_grid_s* g0, const mh_grid_s* g1, int solver, DtSrc dts, const Bc& bc, int ghosts,
-                      unsigned long long* speed, int clock_speed, int pp, int* redo, hipStream_t st) {
-    FusedArgs a;
-    a.dts = dts;
-    a.dx = g0->dx;
-    a.dy = g0->dy;
-    a.seg = seg_fused_for(g0, shape());
-    a.ngroups = g0->d.nx / 64;
-    a.ntiles = a.ngroups * ((g0->d.ny + a.seg - 1) / a.seg);
-    a.ghosts = ghosts;
-    a.bc = bc;
-    a.speed = speed;
-    a.clock_speed = clock_speed;
-    a.redo = redo;
-    a.pp = pp;
-    const View v0 = g0->view(), v1 = g1->view();
-    switch (solver) {
-        case kHLL: go_fused<kHLL>(st, v0, v1, a); break;
-        case kHLLC: go_fused<kHLLC>(st, v0, v1, a); break;
-        case kForce: go_fused<kForce>(st, v0, v1, a); break;
-        default: return fail(MH_ERR_INVALID, "fused step: solver without the fused path");
-    }
-    return check_hip(hipGetLastError(), "fused step launch");
-}
+                      unsigned long long* speed, int clock_speed, int pp, int* redo, hipStream_t st);
 
 int launch_fill(const mh_grid_s* g, const Bc& bc, hipStream_t st) {
     const int n = 4 * g->d.ny + 4 * (g->d.nx + 4);
@@ -853,7 +665,7 @@ int stepper_settle(mh_stepper s, const Clock& c, hipStream_t st) {
 // Reads the clock (synchronising `st`) and settles the state.
 int stepper_sync(mh_stepper s, hipStream_t st, Clock* out) {
     Clock c;
-    hipLaunchKernelGGL(k_clock, dim3(1), dim3(64), 0, st, s->clock, s->dt_log, 1);
+    hipLaunchKernelGGL(k_clock, dim3(1), dim3(64), 0, st, s->clock, s->dt_log, 1, nullptr);
     int rc = check_hip(hipGetLastError(), "clock launch");
     if (!rc) rc = check_hip(hipMemcpyAsync(&c, s->clock, sizeof(c), hipMemcpyDeviceToHost, st), "hipMemcpyAsync");
     if (!rc) rc = check_hip(hipStreamSynchronize(st), "hipStreamSynchronize");
@@ -1135,7 +947,7 @@ int mh_stepper_advance(mh_stepper s, int nsteps, mh_stream stream) {
     if (!s || nsteps < 0) return fail(MH_ERR_INVALID, "mh_stepper_advance: bad argument");
     hipStream_t st = (hipStream_t)stream;
     for (int k = 0; k < nsteps; ++k) {
-        hipLaunchKernelGGL(k_clock, dim3(1), dim3(64), 0, st, s->clock, s->dt_log, 0);
+        hipLaunchKernelGGL(k_clock, dim3(1), dim3(64), 0, st, s->clock, s->dt_log, 0, s->fused ? s->tmp->redo : nullptr);
         int rc = check_hip(hipGetLastError(), "clock launch");
         if (!rc) rc = stepper_step(s, st);
         if (rc) return rc;
@@ -1401,7 +1213,7 @@ int mh_stepper_advance_profiled(mh_stepper s, int nsteps, mh_stream stream, doub
     for (int k = 0; k < nsteps && !rc; ++k) {
         hipEvent_t* e = &ev[(size_t)k * 4];
         rc = check_hip(hipEventRecord(e[0], st), "hipEventRecord");
-        hipLaunchKernelGGL(k_clock, dim3(1), dim3(64), 0, st, s->clock, s->dt_log, 0);
+        hipLaunchKernelGGL(k_clock, dim3(1), dim3(64), 0, st, s->clock, s->dt_log, 0, s->fused ? s->tmp->redo : nullptr);
         if (!rc) rc = check_hip(hipGetLastError(), "clock launch");
         if (!rc) rc = check_hip(hipEventRecord(e[1], st), "hipEventRecord");
         if (s->fused) {
