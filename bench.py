@@ -1,18 +1,26 @@
 """Benchmark: FP64 MUSCL-Hancock HLLC throughput at 8192^2 on one MI355X.
 
 A step is one pass of the hot path over the grid: the device clock (dt from
-the previous step's fused max-speed reduction), the x sweep and the y sweep
-of the dimensionally split second-order scheme.  The workload is the
+the previous step's fused max-speed reduction) and the split step -- on
+this grid the fused x-then-y step kernel (one read and one write of the
+state per step; the two-sweep form with --fused 0).  The workload is the
 cylindrical blast (rho=1, p=1 inside r <= 0.4 of (1,1); rho=0.125, p=0.1
 outside; [0,2]^2) on an 8192 x 8192 grid, HLLC, CFL 0.8, state resident in
 HBM before the timed region.
 
 value: cell updates per second over the timed steps (Gcell-updates/s).
-roofline: the dominant kernel (the slower sweep) -- 64 algorithmic bytes per
-  cell (read 32 B, write 32 B of the four FP64 planes) times nx*ny cells per
-  launch, divided by its average duration from HIP events recorded on the
-  launch stream over the timed region; peak 8 TB/s.  The whole-step fraction
-  (128 B per cell update) is reported beside it.
+roofline: the dominant kernel -- its algorithmic bytes per launch at 128 B
+  per cell update (the split step's x sweep and y sweep each reading and
+  writing the four FP64 planes: the per-unit figure the target is set in)
+  times the cell updates one launch performs (a whole step for the fused
+  kernel, half a step's bytes for one sweep), divided by its average
+  duration from HIP events recorded on the launch stream over the timed
+  region; peak 8 TB/s.  "traffic" is the launch's measured memory traffic
+  (two counter passes, profiles/).  The fused kernel reads and writes the
+  state once, 64 B per cell update; that rate is reported beside it
+  ("kernel_bytes"), and the kernel is bound by FP64 VALU issue rather than
+  by HBM.  The whole-step fraction (128 B per cell update over the step
+  time, clock included) is "step_hbm_frac".
 cpu_baseline: the reference CPU solver compiled from its sources
   (oracle/_ref) -- or the oracle port when that build is absent -- on the
   same workload (8192^2 HLLC) for a bounded sample (median of 3 runs of a
@@ -31,7 +39,8 @@ adjoint: the 512x256 shock-bubble adjoint case (400 HLLC steps, a
   gradient (checkpointing forward + backward) over it.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--size N]
-       [--solver hllc] [--no-cpu] [--no-extras] [--leg batched|adjoint|snapshot]
+       [--solver hllc] [--fused 1|0] [--no-cpu] [--no-extras]
+       [--leg batched|adjoint|snapshot]
 (--leg runs one extra leg alone and prints its JSON object: the profiling
 runs of those workloads use it; "snapshot" measures the cost of the device
 loop's snapshots at --size, "cpu" times the CPU baseline alone at --size
@@ -378,6 +387,8 @@ def main():
     ap.add_argument("--state", choices=["case", "smooth"], default="case",
                     help="case: the cylindrical blast's initial state; smooth: a smooth state with no two equal "
                          "neighbours (no uniform region), on the same grid")
+    ap.add_argument("--fused", type=int, choices=[0, 1, 2], default=1,
+                    help="1: the fused step where it applies on grids of 2^20 cells or more (default); 2: on every grid; 0: the x sweep and the y sweep")
     ap.add_argument("--copy-only", type=int, default=0,
                     help="only run the copy kernel on buffers of this many MiB (counter calibration)")
     args = ap.parse_args()
@@ -406,7 +417,9 @@ def main():
         return
 
     from large_sys_eq_opt_amd import Grid2DGPU, Stepper, cases, parse_riemann_solver
+    from large_sys_eq_opt_amd.solver import set_launch_shape
 
+    set_launch_shape(fused=args.fused)
     solver = parse_riemann_solver(args.solver)
     n = args.size
     if n % 256:
@@ -435,9 +448,13 @@ def main():
     names = ["clock", "step_fused"] if fused else ["clock", "sweep_x", "sweep_y"]
     kern = {names[i]: {"avg_ms": ms_k[i]} for i in range(len(names))}
     for nm in names[1:]:
+        # bytes the launch itself reads and writes: the state once
         kern[nm]["GB_per_s"] = BYTES_PER_SWEEP * cells / (kern[nm]["avg_ms"] * 1e-3) / 1e9
     dom = max(names[1:], key=lambda k: kern[k]["avg_ms"])
-    achieved = BYTES_PER_SWEEP * cells / (kern[dom]["avg_ms"] * 1e-3)
+    # algorithmic bytes per launch at 128 B per cell update: a whole step
+    # for the fused kernel, one sweep's half for a sweep
+    alg_bytes = (2 if fused else 1) * BYTES_PER_SWEEP * cells
+    achieved = alg_bytes / (kern[dom]["avg_ms"] * 1e-3)
     traffic = None
     tfile = os.path.join(ROOT, "profiles", f"traffic_{dom}_{args.solver.lower()}_cyl_blast_{n}.json")
     if args.state == "case" and os.path.exists(tfile):
@@ -458,11 +475,15 @@ def main():
         "data": "synthetic",
         "config": {"workload": (f"cyl_blast {n}x{n}" if args.state == "case" else f"smooth non-uniform state {n}x{n}")
                                + f" {args.solver.upper()} FP64 MUSCL-Hancock, x-then-y split, "
-                               f"device-resident dt (clock + x sweep + y sweep per step)",
+                               + ("device-resident dt (clock + fused x-then-y step per step)" if fused else
+                                  "device-resident dt (clock + x sweep + y sweep per step)"),
                    "nx": cfg.nx, "ny": cfg.ny, "solver": args.solver, "cfl": cfg.cfl, "precision": "fp64"},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK, "traffic": traffic, "kernel": dom,
-                     "algorithmic_bytes_per_launch": BYTES_PER_SWEEP * cells},
+                     "algorithmic_bytes_per_launch": alg_bytes,
+                     "kernel_bytes": {"read_write_per_launch": BYTES_PER_SWEEP * cells,
+                                      "GB_per_s": kern[dom]["GB_per_s"],
+                                      "frac": kern[dom]["GB_per_s"] * 1e9 / HBM_PEAK}},
         "step_hbm_frac": rate * 2 * BYTES_PER_SWEEP / HBM_PEAK,
         "kernels": kern,
         "event_ms_total": ms_total,

@@ -2,7 +2,8 @@
 wavefront through LDS) gives the x sweep followed by the y sweep bit for bit.
 
 Device-loop runs on grids whose width is a multiple of 64 (the fused path's
-domain) against the oracle: ragged heights and chunk sizes (partial last
+domain; the launch shape's fused = 2 takes it below the default's 2^20-cell
+threshold) against the oracle: ragged heights and chunk sizes (partial last
 chunks, chunks shorter than the 32-row window), states on both sides of the
 fast-division range (tiles redone by the IEEE pass), tiny momenta (the HLL /
 HLLC numerators that take the IEEE division), a state so cold that every
@@ -62,7 +63,7 @@ def test_fused_mixed_range_bitwise(hip, oracle, solver, nx, ny, seg):
     cfg = cases.CaseConfig(nx, ny, 2, 0.0, 1.0, 0.0, 1.0, 0.8, 0.0)
     U0 = scaled_field(oracle, nx, ny, 7)
     o = oracle.run(U0, cfg.dx, cfg.dy, solver, cfg.cfl, max_steps=steps)
-    n, dts, out = _run(hip, cfg, solver, U0, steps, dict(fused=1, seg_y=seg, min_tiles=0))
+    n, dts, out = _run(hip, cfg, solver, U0, steps, dict(fused=2, seg_y=seg, min_tiles=0))
     _check(o, n, dts, out, steps)
 
 
@@ -76,7 +77,7 @@ def test_fused_blast_bitwise(hip, oracle, solver):
     kind, params = cases.case_params("cyl_blast")
     U0 = oracle.init(kind, params.array(), 256, 256, cfg.bounds)
     o = oracle.run(U0, cfg.dx, cfg.dy, solver, cfg.cfl, max_steps=steps)
-    n, dts, out = _run(hip, cfg, solver, U0, steps, dict(fused=1))
+    n, dts, out = _run(hip, cfg, solver, U0, steps, dict(fused=2))
     _check(o, n, dts, out, steps)
 
 
@@ -88,7 +89,7 @@ def test_fused_tiny_momenta_bitwise(hip, oracle, solver):
     cfg = cases.CaseConfig(nx, ny, 2, 0.0, 1.0, 0.0, 1.0, 0.8, 0.0)
     U0 = tiny_momentum_field(oracle, nx, ny, 5)
     o = oracle.run(U0, cfg.dx, cfg.dy, solver, cfg.cfl, max_steps=steps)
-    n, dts, out = _run(hip, cfg, solver, U0, steps, dict(fused=1))
+    n, dts, out = _run(hip, cfg, solver, U0, steps, dict(fused=2))
     _check(o, n, dts, out, steps)
 
 
@@ -109,7 +110,7 @@ def test_fused_every_tile_redone(hip, oracle):
     U0 = oracle.prim_to_cons(V.reshape(-1, 4)).reshape(ny + 4, nx + 4, 4)
     o = oracle.run(U0, cfg.dx, cfg.dy, 1, cfg.cfl, max_steps=steps)
     assert o["dt"][0] / cfg.dx > 2.0 ** 20
-    n, dts, out = _run(hip, cfg, 1, U0, steps, dict(fused=1))
+    n, dts, out = _run(hip, cfg, 1, U0, steps, dict(fused=2))
     _check(o, n, dts, out, steps)
 
 
@@ -127,7 +128,7 @@ def test_fused_inflow_bitwise(hip, oracle, solver):
     U0 = oracle.init(kind, params.array(), nx, ny, c.bounds)
     bc = cases.case_boundary(case)
     o = oracle.run(U0, c.dx, c.dy, solver, c.cfl, max_steps=steps, inflow_left=bc.left_inflow_state)
-    n, dts, out = _run(hip, c, solver, U0, steps, dict(fused=1, seg_y=48, min_tiles=0), bc=bc)
+    n, dts, out = _run(hip, c, solver, U0, steps, dict(fused=2, seg_y=48, min_tiles=0), bc=bc)
     _check(o, n, dts, out, steps)
 
 
