@@ -166,7 +166,9 @@ def cpu_baseline_subprocess(solver_name: str, n: int):
 
 MODES = {0: "persistent: one launch per step, grid barrier between the x and y phases",
          1: "persistent: two launches per step (x phase, y phase)",
-         2: "dispatched: two launches per step (x tiles; y tiles + IEEE redo), one tile per wavefront"}
+         2: "dispatched: two launches per step (x tiles; y tiles + IEEE redo), one tile per wavefront",
+         3: "fused: one launch per step (an x/y wavefront pair per y tile, x rows handed to y through LDS) "
+            "+ its IEEE redo and clock commit"}
 
 
 def _timed_batch(spec, schedule, steps, per_graph=25, size_class=(0, 0, 0)):

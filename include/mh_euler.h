@@ -135,8 +135,7 @@ int mh_sweep(mh_grid in, mh_grid out, double dt, int dir, int solver, mh_stream 
  * grid.  t_end <= 0: no clamp, steps run unconditionally.  bc NULL =
  * transmissive.  Fills the ghosts of U and reduces its max signal speed.
  * With the launch shape's `fused` set (by default for grids of 2^20 cells
- * or more) and the fused step applicable (nx a multiple of 64; HLL, HLLC or
- * FORCE), each step is one pass and the state alternates between U and the
+ * or more) and the fused step applicable (HLL, HLLC or FORCE), each step is one pass and the state alternates between U and the
  * scratch grid: it is in U
  * again after mh_stepper_clock, mh_stepper_finish or mh_stepper_destroy,
  * which is when U may be read. */
@@ -180,8 +179,8 @@ typedef struct {
                       tile per workgroup; 0 keeps the shape as set
                       (default 8192) */
     int fused;    /* steppers created from now on run each step as one fused
-                     x-then-y pass where it applies (nx a multiple of 64;
-                     HLL, HLLC, FORCE), with seg_y rows per chunk: 1
+                     x-then-y pass where it applies (HLL, HLLC, FORCE),
+                     with seg_y rows per chunk: 1
                      (default) on grids of at least 2^20 cells, 2 on every
                      such grid; 0: always the x sweep and the y sweep */
 } mh_launch_shape;
@@ -242,8 +241,14 @@ int mh_batch_prepare(mh_batch b, mh_stream stream);
 /* Launch schedule: x tile row segment (0 = whole row) and rows per x tile
  * (>= 1), y tile chunk rows (0 = whole column), mode (0: one persistent
  * launch per step; 1: two persistent launches, x then y; 2: two dispatched
- * launches with one tile per wavefront), persistent workgroups per CU (0 =
- * occupancy limit).  Drops a captured graph. */
+ * launches with one tile per wavefront; 3: the fused step, a wavefront pair
+ * per y tile sweeping x into LDS and y out of it, plus its redo launch --
+ * HLL, HLLC, FORCE; the exact solver runs mode 2 instead), persistent
+ * workgroups per CU (0 = occupancy limit).  Under mode 3 an instance's state
+ * alternates between its grid and its scratch grid; mh_batch_sync,
+ * mh_batch_clock, mh_batch_set_schedule and mh_batch_set_size_class
+ * synchronise and put it back in the grid, so read instance grids after one
+ * of them.  Drops a captured graph. */
 int mh_batch_set_schedule(mh_batch b, int xseg, int xrows, int ychunk, int mode, int blocks_per_cu);
 /* Size class of the schedule: instances with fewer than side_below cells a
  * side (nx) take y chunks of ychunk rows and x tiles of xrows rows instead
@@ -254,8 +259,13 @@ int mh_batch_set_size_class(mh_batch b, int side_below, int ychunk, int xrows);
 int mh_batch_step(mh_batch b, int nsteps, mh_stream stream);
 /* Captures steps_per_graph steps into a hipGraph used by mh_batch_step. */
 int mh_batch_capture(mh_batch b, int steps_per_graph, mh_stream stream);
-/* Synchronises; per-instance time, last dt, steps, status (arrays of n or NULL). */
+/* Synchronises (and puts each instance's state back into its grid after
+ * mode-3 steps); per-instance time, last dt, steps, status (arrays of n or
+ * NULL). */
 int mh_batch_clock(mh_batch b, double* t, double* dt, int* steps, int* status, mh_stream stream);
+/* Synchronises and puts each instance's state back into its grid after
+ * mode-3 steps (no status check): call before reading instance grids. */
+int mh_batch_sync(mh_batch b, mh_stream stream);
 /* Persistent grid size and tile counts of the current schedule. */
 int mh_batch_info(mh_batch b, int* nblocks, int* nxtiles, int* nytiles);
 

@@ -77,6 +77,7 @@ SIGNATURES = {
     "mh_batch_step": ([_P, _I, _P], _I),
     "mh_batch_capture": ([_P, _I, _P], _I),
     "mh_batch_clock": ([_P, _P, _P, _P, _P, _P], _I),
+    "mh_batch_sync": ([_P, _P], _I),
     "mh_batch_info": ([_P, C.POINTER(_I), C.POINTER(_I), C.POINTER(_I)], _I),
     "mh_adjoint_create": ([C.POINTER(GridDesc), _I, C.POINTER(BcDesc), _I, _I, C.POINTER(_P)], _I),
     "mh_adjoint_destroy": ([_P], _I),

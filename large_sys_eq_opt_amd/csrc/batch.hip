@@ -27,8 +27,13 @@
 // the same persistent kernel or of dispatched kernels (k_batch_x, k_batch_y,
 // k_batch_y_redo) with one tile per wavefront: there the hardware dispatcher
 // balances the load as tiles finish, tiles are ordered largest instance
-// first, and each phase gets its own register budget.  The autotuner times
-// the modes.
+// first, and each phase gets its own register budget.  Or (mode 3) it uses
+// the fused split step (fused_tile.h): one launch per step in which a pair
+// of wavefronts per y tile sweeps x into an LDS ring and y out of it, plus
+// the launch that redoes the tiles whose fast pass gave up and commits the
+// clocks; an instance's state then alternates between its grid and its
+// scratch grid (BatchClock::cur) and mh_batch_clock puts it back.  The
+// autotuner times the modes.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -40,6 +45,7 @@
 #include "euler_device.h"
 #include "grid.h"
 #include "sweep_tiles.h"
+#include "fused_tile.h"
 
 namespace mh {
 
@@ -53,6 +59,7 @@ struct BatchClock {
     unsigned long long smax[2];
     double t, dt;
     int step, status;
+    int cur;   // fused steps: the state is in the instance's grid (0) or its scratch grid (1)
 };
 
 struct XTile { int inst, j0, j1, i0, i1; };   // rows j0..j1-1, cells i0..i1-1
@@ -277,6 +284,128 @@ __global__ __launch_bounds__(256) void k_batch_y_redo(BatchArgs a) {
     }
 }
 
+// Values read from an instance's record after the kernel may have stored
+// (the compiler then loads them into vector registers, which the fused
+// tile's HLLC pass cannot spare): the same value in every lane, moved into
+// scalar registers.
+__device__ __forceinline__ int64_t uniform64(int64_t x) {
+    return (int64_t)(((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(x >> 32)) << 32) |
+                     (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)x));
+}
+__device__ __forceinline__ double uniform(double x) {
+    return __longlong_as_double(uniform64(__double_as_longlong(x)));
+}
+__device__ __forceinline__ View uniform(const View& v) {
+    View u;
+    u.o = reinterpret_cast<double*>(uniform64(reinterpret_cast<int64_t>(v.o)));
+    u.ps = uniform64(v.ps);
+    u.rp = uniform64(v.rp);
+    u.nx = __builtin_amdgcn_readfirstlane(v.nx);
+    u.ny = __builtin_amdgcn_readfirstlane(v.ny);
+    return u;
+}
+
+// Fused step (mode 3): workgroup q is the x/y wavefront pair of y tile q
+// (tiles of the largest instances first); each reads its instance's state
+// where the instance's clock says it is and writes the other grid.  A tile
+// whose fast pass gives up goes on the redo list.
+template <int S>
+__global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_batch_fused(BatchArgs a,
+                                                                                              int ntiles) {
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const int q = blockIdx.x;
+    if (q >= ntiles) return;
+    const int par = __hip_atomic_load(&a.ctrl->step, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & 1;
+    const YTile t = a.yt[q];
+    // the views and step factors in scalar registers; the boundary spec,
+    // read only at the grid's edges, stays in memory
+    const BatchInst& I = a.inst[t.inst];
+    const BatchClock& c = a.clk[t.inst];
+    double dt;
+    if (!inst_dt(I, c, par, dt)) return;
+    const bool swap = __builtin_amdgcn_readfirstlane(c.cur) != 0;
+    const View in = uniform(swap ? I.T : I.U), out = uniform(swap ? I.U : I.T);
+    const double dtdx = uniform(dt / I.dx), dtdy = uniform(dt / I.dy);
+    const Bc& bc = I.bc;
+    __shared__ PairLds X;
+    if (!(dt_in_fast_range(dtdx) && dt_in_fast_range(dtdy))) {
+        if (w == 1 && lane == 0) a.redo[2 + atomicAdd(a.redo, 1)] = q;
+        return;
+    }
+    double m = 0.0;
+    const bool ok = pair_side<S, true>(w, in, out, t.cg * 64, t.j0, t.j1, dtdx, dtdy, bc, 1, 1, X, lane, m);
+    if (w == 1) {
+        if (!(ok && X.xok != 0u)) {
+            if (lane == 0) a.redo[2 + atomicAdd(a.redo, 1)] = q;
+        } else {
+            m = wave_max(m);
+            if (lane == 0 && m > 0.0)
+                atomicMax(&a.clk[t.inst].smax[par ^ 1], (unsigned long long)__double_as_longlong(m));
+        }
+    }
+}
+
+// IEEE pass over the tiles k_batch_fused listed (both wavefronts of a
+// workgroup walk the same tiles, a barrier between tiles); then the last
+// workgroup to finish commits the step: every instance that stepped adds its
+// dt, flips its state location and clears the speed slot the step read; the
+// list is emptied and the step counter advanced.
+template <int S>
+__global__ __launch_bounds__(128) void k_batch_fused_redo(BatchArgs a) {
+    __shared__ PairLds X;
+    __shared__ int last;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const int step = __hip_atomic_load(&a.ctrl->step, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const int par = step & 1;
+    const int n = __hip_atomic_load(a.redo, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    for (int r = blockIdx.x; r < n; r += gridDim.x) {
+        const YTile t = a.yt[a.redo[2 + r]];
+        const BatchInst& I = a.inst[t.inst];
+        const BatchClock& c = a.clk[t.inst];
+        double dt;
+        if (inst_dt(I, c, par, dt)) {
+            const bool swap = __builtin_amdgcn_readfirstlane(c.cur) != 0;
+            const View in = uniform(swap ? I.T : I.U), out = uniform(swap ? I.U : I.T);
+            const double dtdx = uniform(dt / I.dx), dtdy = uniform(dt / I.dy);
+            double m = 0.0;
+            pair_side<S, false>(w, in, out, t.cg * 64, t.j0, t.j1, dtdx, dtdy, I.bc, 1, 1, X, lane, m);
+            if (w == 1) {
+                m = wave_max(m);
+                if (lane == 0 && m > 0.0)
+                    atomicMax(&a.clk[t.inst].smax[par ^ 1], (unsigned long long)__double_as_longlong(m));
+            }
+        }
+        pair_barrier();
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        last = atomicAdd(a.redo + 1, 1) == (int)gridDim.x - 1;
+    }
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    for (int k = threadIdx.x; k < a.ninst; k += blockDim.x) {
+        BatchClock& c = a.clk[k];
+        double dt;
+        if (inst_dt(a.inst[k], c, par, dt)) {
+            c.dt = dt;
+            c.t += dt;
+            c.step += 1;
+            c.cur ^= 1;
+        } else {
+            c.status = MH_ERR_WAVESPEED;
+        }
+        c.smax[par] = 0ull;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        a.redo[0] = 0;
+        a.redo[1] = 0;
+        __hip_atomic_store(&a.ctrl->step, step + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
 // Initial max signal speed of every instance into slot 0.
 __global__ __launch_bounds__(256) void k_batch_speed(const BatchInst* inst, BatchClock* clk, int ninst) {
     const int k = blockIdx.y;
@@ -316,7 +445,8 @@ struct mh_batch_s {
     int nblocks = 0;
     int nxt = 0, nyt = 0;
     // schedule; mode 0: one persistent launch per step, 1: two persistent
-    // launches, 2: two dispatched launches (one tile per wavefront)
+    // launches, 2: two dispatched launches (one tile per wavefront), 3: the
+    // fused step (a wavefront pair per y tile; the exact solver takes mode 2)
     int xseg = 0, xrows = 1, ychunk = 256, mode = 0, blocks_per_cu = 0;
     // size class: instances with fewer than small_side cells a side take
     // their own y chunk rows and x tile rows (0: the schedule's)
@@ -419,7 +549,7 @@ int build_schedule(mh_batch b) {
                 yc.push_back(64.0 * (j1 - j0 + 4));
             }
     }
-    if (b->mode == 2) {
+    if (b->mode >= 2) {
         // Dispatched launches: longest tiles first (each instance's short
         // remainder tiles end up in the tail, where they fill idle SIMDs).
         auto lpt = [](auto& tiles, std::vector<double>& cost) {
@@ -465,7 +595,19 @@ int launch_step(mh_batch b, hipStream_t st) {
     a.ctrl = b->dctrl;
     a.redo = b->dredo;
     const dim3 g(b->nblocks), blk(256);
-    if (b->mode == 2) {
+    if (b->mode == 3 && b->solver != kExact) {
+        const dim3 gf((unsigned)std::max(1, b->nyt)), gr((unsigned)std::max(1, std::min(256, b->nyt))), bf(128);
+        switch (b->solver) {
+            case kHLL: hipLaunchKernelGGL(k_batch_fused<kHLL>, gf, bf, 0, st, a, b->nyt);
+                       hipLaunchKernelGGL(k_batch_fused_redo<kHLL>, gr, bf, 0, st, a); break;
+            case kHLLC: hipLaunchKernelGGL(k_batch_fused<kHLLC>, gf, bf, 0, st, a, b->nyt);
+                        hipLaunchKernelGGL(k_batch_fused_redo<kHLLC>, gr, bf, 0, st, a); break;
+            default: hipLaunchKernelGGL(k_batch_fused<kForce>, gf, bf, 0, st, a, b->nyt);
+                     hipLaunchKernelGGL(k_batch_fused_redo<kForce>, gr, bf, 0, st, a); break;
+        }
+        return check_hip(hipGetLastError(), "batch step launch");
+    }
+    if (b->mode >= 2) {
         const dim3 gx((unsigned)std::max(1, b->nxt)), gy((unsigned)std::max(1, (b->nyt + 3) / 4)), bx(64);
         const dim3 gr((unsigned)std::max(1, std::min(64, (b->nyt + 3) / 4)));
         switch (b->solver) {
@@ -495,6 +637,44 @@ int launch_step(mh_batch b, hipStream_t st) {
         }
     }
     return check_hip(hipGetLastError(), "batch step launch");
+}
+
+// Puts every instance's state back into its grid after fused steps (an
+// instance whose clock says cur = 1 has it in its scratch grid: the two
+// grids swap roles) and leaves the clocks as the other modes expect them
+// between steps: cur = 0 and the speed slot the next step fills cleared.
+// Synchronises `st`.
+int batch_settle(mh_batch b, hipStream_t st, std::vector<BatchClock>* out_clk, Ctrl* out_ctrl) {
+    std::vector<BatchClock> clk(b->n);
+    Ctrl c;
+    int rc = check_hip(hipMemcpyAsync(clk.data(), b->dclk, sizeof(BatchClock) * b->n, hipMemcpyDeviceToHost, st), "hipMemcpyAsync");
+    if (!rc) rc = check_hip(hipMemcpyAsync(&c, b->dctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, st), "hipMemcpyAsync");
+    if (!rc) rc = check_hip(hipStreamSynchronize(st), "hipStreamSynchronize");
+    if (rc) return rc;
+    bool moved = false;
+    const int nxt = (c.step & 1) ^ 1;
+    for (int k = 0; k < b->n; ++k) {
+        if (clk[k].smax[nxt] != 0ull) {
+            clk[k].smax[nxt] = 0ull;
+            moved = true;
+        }
+        if (!clk[k].cur) continue;
+        BatchInst& I = b->hinst[k];
+        std::swap(I.U, I.T);
+        b->grids[k].base = I.U.o - ((int64_t)kNg * I.U.rp + kLead);
+        clk[k].cur = 0;
+        moved = true;
+    }
+    if (moved) {
+        rc = check_hip(hipMemcpyAsync(b->dinst, b->hinst.data(), sizeof(BatchInst) * b->n, hipMemcpyHostToDevice, st),
+                       "hipMemcpyAsync");
+        if (!rc) rc = check_hip(hipMemcpyAsync(b->dclk, clk.data(), sizeof(BatchClock) * b->n, hipMemcpyHostToDevice, st),
+                                "hipMemcpyAsync");
+        if (!rc) rc = check_hip(hipStreamSynchronize(st), "hipStreamSynchronize");
+    }
+    if (out_clk) *out_clk = clk;
+    if (out_ctrl) *out_ctrl = c;
+    return rc;
 }
 
 }  // namespace
@@ -604,8 +784,10 @@ int mh_batch_prepare(mh_batch b, mh_stream stream) {
 }
 
 int mh_batch_set_schedule(mh_batch b, int xseg, int xrows, int ychunk, int mode, int blocks_per_cu) {
-    if (!b || xseg < 0 || xrows < 1 || ychunk < 0 || blocks_per_cu < 0 || mode < 0 || mode > 2)
+    if (!b || xseg < 0 || xrows < 1 || ychunk < 0 || blocks_per_cu < 0 || mode < 0 || mode > 3)
         return fail(MH_ERR_INVALID, "mh_batch_set_schedule: bad argument");
+    const int rc = batch_settle(b, nullptr, nullptr, nullptr);
+    if (rc) return rc;
     b->xseg = xseg;
     b->xrows = xrows;
     b->ychunk = ychunk;
@@ -616,6 +798,8 @@ int mh_batch_set_schedule(mh_batch b, int xseg, int xrows, int ychunk, int mode,
 
 int mh_batch_set_size_class(mh_batch b, int side_below, int ychunk, int xrows) {
     if (!b || side_below < 0 || ychunk < 0 || xrows < 0) return fail(MH_ERR_INVALID, "mh_batch_set_size_class: bad argument");
+    const int rc = batch_settle(b, nullptr, nullptr, nullptr);
+    if (rc) return rc;
     b->small_side = side_below;
     b->small_ychunk = ychunk;
     b->small_xrows = xrows;
@@ -665,11 +849,9 @@ int mh_batch_capture(mh_batch b, int steps_per_graph, mh_stream stream) {
 int mh_batch_clock(mh_batch b, double* t, double* dt, int* steps, int* status, mh_stream stream) {
     if (!b) return fail(MH_ERR_INVALID, "mh_batch_clock: null batch");
     hipStream_t st = (hipStream_t)stream;
-    std::vector<BatchClock> clk(b->n);
+    std::vector<BatchClock> clk;
     Ctrl c;
-    int rc = check_hip(hipMemcpyAsync(clk.data(), b->dclk, sizeof(BatchClock) * b->n, hipMemcpyDeviceToHost, st), "hipMemcpyAsync");
-    if (!rc) rc = check_hip(hipMemcpyAsync(&c, b->dctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, st), "hipMemcpyAsync");
-    if (!rc) rc = check_hip(hipStreamSynchronize(st), "hipStreamSynchronize");
+    const int rc = batch_settle(b, st, &clk, &c);
     if (rc) return rc;
     int bad = 0;
     for (int k = 0; k < b->n; ++k) {
@@ -682,6 +864,11 @@ int mh_batch_clock(mh_batch b, double* t, double* dt, int* steps, int* status, m
     if (c.err) return fail(MH_ERR_LAUNCH, "batch: grid barrier timed out (workgroups not co-resident)");
     if (bad) return fail(MH_ERR_WAVESPEED, "batch: an instance met a non-positive max wave speed");
     return MH_OK;
+}
+
+int mh_batch_sync(mh_batch b, mh_stream stream) {
+    if (!b) return fail(MH_ERR_INVALID, "mh_batch_sync: null batch");
+    return batch_settle(b, (hipStream_t)stream, nullptr, nullptr);
 }
 
 int mh_batch_info(mh_batch b, int* nblocks, int* nxtiles, int* nytiles) {

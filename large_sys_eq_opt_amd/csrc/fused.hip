@@ -48,13 +48,8 @@ __device__ __forceinline__ void fused_tile(const View& in, const View& out, cons
     const int j0 = (tile / a.ngroups) * a.seg;
     const int j1 = min(j0 + a.seg, in.ny);
     const int want = (a.speed || a.clock_speed) ? 1 : 0;
-    if (w == 0) {
-        pair_x<S, F>(in, cg * 64, j0, j1, dtdx, 0.5 * dtdx, a.bc, X, lane);
-        ok = true;
-    } else {
-        ok = pair_y<S, F>(out, cg * 64, j0, j1, dtdy, 0.5 * dtdy, a.bc, a.ghosts, want, X, lane, m);
-        ok = ok && X.xok != 0u;
-    }
+    ok = pair_side<S, F>(w, in, out, cg * 64, j0, j1, dtdx, dtdy, a.bc, a.ghosts, want, X, lane, m);
+    if (w == 1) ok = ok && X.xok != 0u;
 }
 
 template <int S>
@@ -112,16 +107,15 @@ __global__ __launch_bounds__(128) void k_step_fused_redo(View g0, View g1, Fused
     }
 }
 
-// The fused step applies to grids whose rows split into whole 64-column
-// strips, with the solvers that have the fast division path.
-bool fused_applies(const mh_grid_s* g, int solver) { return solver != kExact && g->d.nx % 64 == 0; }
+// The fused step applies with the solvers that have the fast division path.
+bool fused_applies(const mh_grid_s*, int solver) { return solver != kExact; }
 
 // Rows per chunk of the fused step: the configured y chunk, shortened (to
 // no fewer than 16 rows: each chunk recomputes 4 warm-up rows) while the
 // grid gives fewer pairs than min_tiles / 2 (the pairs hold two wavefronts
 // each; by default 4096, two rounds of the eight a CU holds).
 int seg_fused_for(const mh_grid_s* g, const mh_launch_shape& sh) {
-    const int ngroups = g->d.nx / 64, ny = g->d.ny;
+    const int ngroups = (g->d.nx + 63) / 64, ny = g->d.ny;
     int seg = sh.seg_y > 0 ? sh.seg_y : ny;
     const int want = sh.min_tiles / 2;
     if ((int64_t)ngroups * ((ny + seg - 1) / seg) < want && seg > 16) {
@@ -133,7 +127,7 @@ int seg_fused_for(const mh_grid_s* g, const mh_launch_shape& sh) {
 
 int fused_tiles(const mh_grid_s* g) {
     const int seg = seg_fused_for(g, shape());
-    return (g->d.nx / 64) * ((g->d.ny + seg - 1) / seg);
+    return ((g->d.nx + 63) / 64) * ((g->d.ny + seg - 1) / seg);
 }
 
 template <int S>
@@ -152,7 +146,7 @@ int launch_step_fused(const mh_grid_s* g0, const mh_grid_s* g1, int solver, DtSr
     a.dx = g0->dx;
     a.dy = g0->dy;
     a.seg = seg_fused_for(g0, shape());
-    a.ngroups = g0->d.nx / 64;
+    a.ngroups = (g0->d.nx + 63) / 64;
     a.ntiles = a.ngroups * ((g0->d.ny + a.seg - 1) / a.seg);
     a.ghosts = ghosts;
     a.bc = bc;

@@ -23,8 +23,16 @@
 //
 // Ghost rows of the intermediate state (rows -2, -1, ny, ny+1 that the y
 // update reads) are the x update of row 0 / ny-1 (transmissive) or the
-// inflow state, as the x sweep's epilogue writes them.  Strips are whole:
-// the grid's nx is a multiple of 64.
+// inflow state, as the x sweep's epilogue writes them.
+//
+// The last strip of a grid whose nx is not a multiple of 64 is partial (P):
+// its W = nx - c0 < 64 columns are lanes 0..W-1.  Row loads clamp at the
+// last ghost cell nx+1, so lanes W and W+1 hold the ghost cells nx and nx+1
+// and give lane W-1 its right neighbour's values in the row pass (for W = 63
+// the window's right side supplies W(nx+1)); lanes past W compute on copies
+// of ghost states and the y wavefront neither stores nor counts them.  Full
+// strips take the code without these clamps and masks (P = false): it keeps
+// the register schedule of the hot path.
 #pragma once
 
 #include "sweep_tiles.h"
@@ -59,19 +67,22 @@ __device__ __forceinline__ void pair_barrier() {
 // lane 32+i loads cells c0+62 .. c0+65: W(c0+64) and the left edge state of
 // cell c0+64.  B rows: 0-3 W, 4-7 flux (left) / conserved edge state
 // (right), 8-11 the edge state's rcp_nr of its density (fast pass) or its
-// density, then u, v, p (right).
-template <int S, bool F>
+// density, then u, v, p (right).  In a partial strip the right side loads
+// from column min(c0 + 62, nx - 1) and its fourth column clamps at nx+1:
+// every load stays in the row, and for W = 63 the values of B are the full
+// strip's (for fewer columns no lane that counts reads them).
+template <int S, bool F, bool P>
 __device__ __forceinline__ void pair_x_window(const View& in, int c0, int j0, int k0, double half, PairLds& X,
                                               int lane, unsigned& key) {
     int it = 0;
     const int i = lane & 31;
     const bool right = lane >= 32;
     const int src = min(max(j0 - 2 + k0 + i, 0), in.ny - 1);
-    const int cs = right ? c0 + 62 : c0 - 2;
+    const int cs = right ? (P ? min(c0 + 62, in.nx - 1) : c0 + 62) : c0 - 2;
     Prim Wq[4];
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
-        const Cons U = ld(in, cs + t, src);
+        const Cons U = ld(in, P && t == 3 ? min(cs + t, in.nx + 1) : cs + t, src);
         Wq[t] = c2p<F>(U);
         if constexpr (F) key = max(key, range_key(U.r, U.e, Wq[t].p));
     }
@@ -100,20 +111,20 @@ __device__ __forceinline__ void pair_x_window(const View& in, int c0, int j0, in
 // (B); the LDS address is selected once per exchange, the components follow
 // at the same stride (66 doubles) in M and B.  Writes the tile's fast-range
 // verdict (F) into X.xok before the last barrier.
-template <int S, bool F>
+template <int S, bool F, bool P>
 __device__ __forceinline__ void pair_x(const View& in, int c0, int j0, int j1, double dtd, double half, const Bc& bc,
                                        PairLds& X, int lane) {
     int it = 0;
     unsigned key = 0;
     const int nrows = j1 - j0 + 4;
-    const uint32_t lane8 = (uint32_t)lane * (uint32_t)sizeof(double);
+    const uint32_t lane8 = (uint32_t)(P ? min(c0 + lane, in.nx + 1) - c0 : lane) * (uint32_t)sizeof(double);
     auto src_row = [&](int k) { return min(max(j0 - 2 + k, 0), in.ny - 1); };
     Cons Un = ld_off(in, row_at(in, c0, src_row(0)), lane8);
     const double* mL = &X.M[0][lane];
     const double* mR = &X.M[0][lane + 2];
     for (int k = 0; k < nrows; ++k) {
         const int w = k & (kPairWindow - 1);
-        if (w == 0) pair_x_window<S, F>(in, c0, j0, k, half, X, lane, key);
+        if (w == 0) pair_x_window<S, F, P>(in, c0, j0, k, half, X, lane, key);
         const Cons U0 = Un;
         Un = ld_off(in, row_at(in, c0, src_row(min(k + 1, nrows - 1))), lane8);
         const Prim W0 = c2p<F>(U0);
@@ -165,12 +176,13 @@ __device__ __forceinline__ void pair_x(const View& in, int c0, int j0, int j1, d
 // update; the update of row p-2 reads that row's intermediate state from
 // the ring too.  Returns the y rows' fast-range verdict (F); m receives the
 // lane's maximum signal speed of the new states when `speed` is set.
-template <int S, bool F>
+template <int S, bool F, bool P>
 __device__ __forceinline__ bool pair_y(const View& out, int c0, int j0, int j1, double dtd, double half, const Bc& bc,
                                        int ghosts, int speed, PairLds& X, int lane, double& m) {
     int it = 0;
     const int i = c0 + lane;
-    const uint32_t so = (uint32_t)(i - kSinkCol) * (uint32_t)sizeof(double);
+    // stores: the lane's cell, or the sink column for a lane past the row's end
+    const uint32_t so = (uint32_t)(!P || i < out.nx ? i - kSinkCol : 0) * (uint32_t)sizeof(double);
     auto held = [&](int r) {
         const double* q = &X.ring[r & 3][0][lane];
         return Cons{q[0], q[64], q[128], q[192]};
@@ -223,8 +235,29 @@ __device__ __forceinline__ bool pair_y(const View& out, int c0, int j0, int j1, 
         Rp = R;
         W1 = W0;
     }
-    if constexpr (F) return !__any(*key >= kRangeKeyEnd);
+    // lanes past a partial strip's last column computed on ghost copies:
+    // their speeds and keys do not count
+    const bool active = !P || i < out.nx;
+    if (!active) m = 0.0;
+    if constexpr (F) return !__any(active && *key >= kRangeKeyEnd);
     return true;
+}
+
+// Both wavefronts' side of a pair, with the strip's kind (full or partial)
+// chosen per tile: w = 0 sweeps x into the ring, w = 1 sweeps y out of it
+// and returns its verdict (the x wavefront returns true).
+template <int S, bool F>
+__device__ __forceinline__ bool pair_side(int w, const View& in, const View& out, int c0, int j0, int j1, double dtdx,
+                                          double dtdy, const Bc& bc, int ghosts, int speed, PairLds& X, int lane,
+                                          double& m) {
+    const bool partial = c0 + 64 > in.nx;
+    if (w == 0) {
+        if (partial) pair_x<S, F, true>(in, c0, j0, j1, dtdx, 0.5 * dtdx, bc, X, lane);
+        else pair_x<S, F, false>(in, c0, j0, j1, dtdx, 0.5 * dtdx, bc, X, lane);
+        return true;
+    }
+    if (partial) return pair_y<S, F, true>(out, c0, j0, j1, dtdy, 0.5 * dtdy, bc, ghosts, speed, X, lane, m);
+    return pair_y<S, F, false>(out, c0, j0, j1, dtdy, 0.5 * dtdy, bc, ghosts, speed, X, lane, m);
 }
 
 }  // namespace mh

@@ -95,7 +95,9 @@ class Batch:
         """x tiles: row segments of xseg cells (0 = whole rows), xrows rows
         each; y tiles: chunks of ychunk rows; mode 0: one persistent launch
         per step, 1: two persistent launches, 2: two dispatched launches with
-        one tile per wavefront."""
+        one tile per wavefront, 3: the fused step (one x/y wavefront pair per
+        y tile; instance states alternate between their grids and scratch
+        grids and are back in the grids after sync(), clock() or download())."""
         call("mh_batch_set_schedule", self._h, int(xseg), int(xrows), int(ychunk), int(mode), int(blocks_per_cu))
         self.schedule = (int(xseg), int(ychunk), int(mode), int(blocks_per_cu), int(xrows))
 
@@ -123,7 +125,13 @@ class Batch:
         call("mh_batch_info", self._h, C.byref(a), C.byref(b), C.byref(c))
         return {"blocks": a.value, "x_tiles": b.value, "y_tiles": c.value}
 
+    def sync(self, stream=None):
+        """Waits for the enqueued steps and puts every instance's state in its
+        grid (fused steps leave some in their scratch grids)."""
+        call("mh_batch_sync", self._h, _stream(stream))
+
     def download(self, k: int):
+        self.sync()
         return self.grid(k).download_to_aos()
 
     def close(self):
@@ -147,6 +155,7 @@ DEFAULT_CANDIDATES = [
     # (xseg, ychunk, mode, blocks_per_cu, xrows)
     (0, 256, 0, 0, 1), (0, 128, 1, 0, 1), (0, 256, 2, 0, 4), (0, 192, 2, 0, 4), (0, 128, 2, 0, 4),
     (0, 96, 2, 0, 4), (0, 64, 2, 0, 4), (0, 128, 2, 0, 1), (0, 128, 2, 0, 8),
+    (0, 128, 3, 0, 1), (0, 96, 3, 0, 1), (0, 64, 3, 0, 1), (0, 48, 3, 0, 1), (0, 32, 3, 0, 1),
 ]
 
 
@@ -199,7 +208,7 @@ def autotune(batch: Batch, candidates=None, steps: int = 8, cache: str | None = 
     timings = sorted((t, c) for c, t in best_dt.items())
     best = timings[0][1]
     cls_dt = {(0, 0, 0): timings[0][0]}
-    if best[2] == 2:
+    if best[2] >= 2:
         batch.set_schedule(*best)
         for _ in range(max(1, rounds)):
             for cls in (size_classes or SIZE_CLASSES):

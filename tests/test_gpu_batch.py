@@ -1,6 +1,7 @@
 """Batched ensemble: each instance equals a single-grid run of the same
-initial state bit for bit (and the oracle), under every schedule and when the
-steps replay from a captured hipGraph."""
+initial state bit for bit (and the oracle), under every schedule (the fused
+mode 3 included: most instance sides end in a partial 64-column strip) and
+when the steps replay from a captured hipGraph."""
 from __future__ import annotations
 
 import numpy as np
@@ -27,7 +28,8 @@ def oracle_instance(oracle, s, solver, steps):
 # (xseg, ychunk, mode, blocks_per_cu, xrows): persistent fused / two-launch,
 # dispatched, ragged segments and chunks, several rows per x tile
 @pytest.mark.parametrize("schedule", [(0, 256, 0, 0, 1), (0, 32, 1, 0, 1), (48, 0, 0, 1, 2), (0, 64, 2, 0, 1),
-                                      (60, 16, 2, 0, 3), (0, 64, 2, 0, 8), (0, 128, 2, 0, 1000), (0, 96, 1, 0, 7)])
+                                      (60, 16, 2, 0, 3), (0, 64, 2, 0, 8), (0, 128, 2, 0, 1000), (0, 96, 1, 0, 7),
+                                      (0, 128, 3, 0, 1), (0, 37, 3, 0, 1), (0, 0, 3, 0, 1)])
 def test_batch_instances_match_oracle(hip, oracle, schedule):
     from large_sys_eq_opt_amd.ensemble import Batch
 
@@ -46,7 +48,7 @@ def test_batch_instances_match_oracle(hip, oracle, schedule):
 
 
 @pytest.mark.parametrize("schedule, size_class", [((0, 64, 2, 0, 4), (100, 16, 3)), ((0, 256, 0, 0, 1), (1000, 0, 5)),
-                                                  ((60, 128, 2, 0, 8), (50, 40, 0))])
+                                                  ((60, 128, 2, 0, 8), (50, 40, 0)), ((0, 64, 3, 0, 1), (100, 16, 0))])
 def test_batch_size_class_matches_oracle(hip, oracle, schedule, size_class):
     """Instances below a side take their own y chunks and x tile rows (the
     autotuner's size classes): every instance still equals its own oracle
@@ -74,14 +76,17 @@ def sum_seq(dts):
     return t
 
 
-def test_batch_graph_replay_matches_direct_launches(hip, oracle):
+@pytest.mark.parametrize("mode", [0, 3])
+def test_batch_graph_replay_matches_direct_launches(hip, oracle, mode):
     from large_sys_eq_opt_amd.ensemble import Batch
 
     a = Batch(SPEC, solver=1)
+    a.set_schedule(0, 64, mode, 0, 1)
     a.initialise()
     a.capture(4)
     a.step(STEPS)          # 3 graph replays of 4 steps
     b = Batch(SPEC, solver=1)
+    b.set_schedule(0, 64, mode, 0, 1)
     b.initialise()
     b.step(STEPS)          # direct launches
     ta, _, sa, _ = a.clock()
@@ -93,12 +98,16 @@ def test_batch_graph_replay_matches_direct_launches(hip, oracle):
     b.close()
 
 
+@pytest.mark.parametrize("mode", [0, 3])
 @pytest.mark.parametrize("solver", [0, 2, 3])
-def test_batch_other_solvers(hip, oracle, solver):
+def test_batch_other_solvers(hip, oracle, solver, mode):
+    """HLL, exact (within 1e-10: its Newton iteration is not bitwise) and
+    FORCE; under mode 3 the exact solver takes the dispatched launches."""
     from large_sys_eq_opt_amd.ensemble import Batch
 
     spec = SPEC[:4]
     b = Batch(spec, solver=solver)
+    b.set_schedule(0, 64, mode, 0, 1)
     b.initialise()
     b.step(6)
     for k, s in enumerate(spec):
@@ -126,7 +135,7 @@ def test_batch_ensemble_512_seed0_500_steps(hip, oracle):
     rng = np.random.default_rng(3)
     picks = sorted({int(np.argmin(sides)), int(np.argmax(sides)), *rng.choice(len(spec), 6, replace=False).tolist()})
     ref = {k: oracle_instance(oracle, spec[k], 1, 500) for k in picks}
-    for schedule in [(0, 128, 2, 0, 4), (0, 256, 0, 0, 1)]:
+    for schedule in [(0, 128, 2, 0, 4), (0, 256, 0, 0, 1), (0, 64, 3, 0, 1)]:
         b = Batch(spec, solver=1)
         b.set_schedule(*schedule)
         b.initialise()
@@ -139,3 +148,42 @@ def test_batch_ensemble_512_seed0_500_steps(hip, oracle):
             assert same_bits(interior(b.download(k)), interior(o["U"])), (schedule, k, spec[k])
             assert dt[k] == o["dt"][-1] and t[k] == np.float64(sum_seq(o["dt"])), (schedule, k)
         b.close()
+
+
+def test_batch_fused_mode_switches(hip, oracle):
+    """Five fused steps (the instances' states end in their scratch grids),
+    a switch to the persistent schedule (which puts them back), seven more
+    steps, a switch back to the fused mode and three more: every instance
+    equals its 15-step oracle run."""
+    from large_sys_eq_opt_amd.ensemble import Batch
+
+    b = Batch(SPEC, solver=1)
+    b.set_schedule(0, 64, 3, 0, 1)
+    b.initialise()
+    b.step(5)
+    b.set_schedule(0, 256, 0, 0, 1)
+    b.step(7)
+    b.set_schedule(0, 48, 3, 0, 1)
+    b.step(3)
+    t, dt, steps, status = b.clock()
+    assert (steps == 15).all() and (status == 0).all()
+    for k, s in enumerate(SPEC):
+        o = oracle_instance(oracle, s, 1, 15)
+        assert same_bits(interior(b.download(k)), interior(o["U"])), (k, s)
+        assert dt[k] == o["dt"][-1] and t[k] == np.float64(sum_seq(o["dt"]))
+    b.close()
+
+
+def test_batch_fused_download_without_clock(hip, oracle):
+    """download() after an odd number of fused steps, with no clock() call
+    in between, reads the current state."""
+    from large_sys_eq_opt_amd.ensemble import Batch
+
+    b = Batch(SPEC[:3], solver=1)
+    b.set_schedule(0, 64, 3, 0, 1)
+    b.initialise()
+    b.step(3)
+    for k, s in enumerate(SPEC[:3]):
+        o = oracle_instance(oracle, s, 1, 3)
+        assert same_bits(interior(b.download(k)), interior(o["U"])), (k, s)
+    b.close()

@@ -1,10 +1,11 @@
 """The fused split step (one pass per step: an x wavefront feeding a y
 wavefront through LDS) gives the x sweep followed by the y sweep bit for bit.
 
-Device-loop runs on grids whose width is a multiple of 64 (the fused path's
-domain; the launch shape's fused = 2 takes it below the default's 2^20-cell
-threshold) against the oracle: ragged heights and chunk sizes (partial last
-chunks, chunks shorter than the 32-row window), states on both sides of the
+Device-loop runs against the oracle (the launch shape's fused = 2 takes the
+fused step below the default's 2^20-cell threshold): ragged heights and chunk
+sizes (partial last chunks, chunks shorter than the 32-row window), widths
+that end in a partial 64-column strip (1, 2, 62 and 63 columns, and grids
+narrower than one strip), states on both sides of the
 fast-division range (tiles redone by the IEEE pass), tiny momenta (the HLL /
 HLLC numerators that take the IEEE division), a state so cold that every
 tile is redone, and inflow boundaries; the stepper's state alternates
@@ -62,6 +63,23 @@ def test_fused_mixed_range_bitwise(hip, oracle, solver, nx, ny, seg):
     steps = 11
     cfg = cases.CaseConfig(nx, ny, 2, 0.0, 1.0, 0.0, 1.0, 0.8, 0.0)
     U0 = scaled_field(oracle, nx, ny, 7)
+    o = oracle.run(U0, cfg.dx, cfg.dy, solver, cfg.cfl, max_steps=steps)
+    n, dts, out = _run(hip, cfg, solver, U0, steps, dict(fused=2, seg_y=seg, min_tiles=0))
+    _check(o, n, dts, out, steps)
+
+
+@pytest.mark.parametrize("solver", BITWISE_SOLVERS)
+@pytest.mark.parametrize("nx,ny,seg", [(65, 40, 16), (66, 33, 0), (126, 70, 0), (127, 90, 37), (300, 131, 64),
+                                       (30, 50, 0), (1, 20, 0), (63, 64, 32)])
+def test_fused_partial_strips_bitwise(hip, oracle, solver, nx, ny, seg):
+    """Widths that are not a multiple of 64: the last strip's lanes past the
+    grid's edge hold the ghost cells and compute on copies of them; seven
+    steps of the two-blob field."""
+    from large_sys_eq_opt_amd import cases
+
+    steps = 7
+    cfg = cases.CaseConfig(nx, ny, 2, 0.0, 1.0, 0.0, 1.0, 0.8, 0.0)
+    U0 = scaled_field(oracle, nx, ny, 11)
     o = oracle.run(U0, cfg.dx, cfg.dy, solver, cfg.cfl, max_steps=steps)
     n, dts, out = _run(hip, cfg, solver, U0, steps, dict(fused=2, seg_y=seg, min_tiles=0))
     _check(o, n, dts, out, steps)
@@ -129,6 +147,23 @@ def test_fused_inflow_bitwise(hip, oracle, solver):
     bc = cases.case_boundary(case)
     o = oracle.run(U0, c.dx, c.dy, solver, c.cfl, max_steps=steps, inflow_left=bc.left_inflow_state)
     n, dts, out = _run(hip, c, solver, U0, steps, dict(fused=2, seg_y=48, min_tiles=0), bc=bc)
+    _check(o, n, dts, out, steps)
+
+
+def test_fused_inflow_partial_strip_bitwise(hip, oracle):
+    """The shock-bubble case on 200x100 (a last strip of 8 columns), HLLC,
+    20 steps."""
+    from large_sys_eq_opt_amd import cases
+
+    steps, case = 20, "shock_bubble_nd"
+    cfg = cases.get_case_config(case)
+    kind, params = cases.case_params(case)
+    nx, ny = 200, 100
+    c = cases.CaseConfig(nx, ny, 2, cfg.x_min, cfg.x_max, cfg.y_min, cfg.y_max, cfg.cfl, 0.0)
+    U0 = oracle.init(kind, params.array(), nx, ny, c.bounds)
+    bc = cases.case_boundary(case)
+    o = oracle.run(U0, c.dx, c.dy, 1, c.cfl, max_steps=steps, inflow_left=bc.left_inflow_state)
+    n, dts, out = _run(hip, c, 1, U0, steps, dict(fused=2, seg_y=0, min_tiles=0), bc=bc)
     _check(o, n, dts, out, steps)
 
 
