@@ -29,7 +29,7 @@ cpu_baseline: the reference CPU solver compiled from its sources
 batched: 512 blast-wave instances (seed 0, sides log-uniform in [64,1024],
   95.3 M cells), 500 steps with per-instance dt replayed from captured
   hipGraphs under the autotuner's schedule (the line names it: "mode" is
-  dispatched launches or the persistent kernel), its cell-update rate and
+  the fused step, dispatched launches or the persistent kernel), its cell-update rate and
   ratio to the single-grid value; the persistent kernel's best schedule (one
   launch per step with a grid barrier between the phases, or two launches)
   is timed over the same 500 steps beside it ("persistent").

@@ -146,7 +146,7 @@ class Batch:
             pass
 
 
-# Size classes tried on the best dispatched schedule: (side below which an
+# Size classes tried on the best dispatched or fused schedule: (side below which an
 # instance takes them, y chunk rows, x tile rows); (0, 0, 0) is none.
 SIZE_CLASSES = [(0, 0, 0)] + [(side, yc, xr) for side in (256, 512)
                               for yc, xr in ((32, 8), (64, 4), (64, 8), (128, 8), (32, 16), (256, 8), (0, 16))]
@@ -155,7 +155,7 @@ DEFAULT_CANDIDATES = [
     # (xseg, ychunk, mode, blocks_per_cu, xrows)
     (0, 256, 0, 0, 1), (0, 128, 1, 0, 1), (0, 256, 2, 0, 4), (0, 192, 2, 0, 4), (0, 128, 2, 0, 4),
     (0, 96, 2, 0, 4), (0, 64, 2, 0, 4), (0, 128, 2, 0, 1), (0, 128, 2, 0, 8),
-    (0, 128, 3, 0, 1), (0, 96, 3, 0, 1), (0, 64, 3, 0, 1), (0, 48, 3, 0, 1), (0, 32, 3, 0, 1),
+    (0, 256, 3, 0, 1), (0, 192, 3, 0, 1), (0, 128, 3, 0, 1), (0, 96, 3, 0, 1), (0, 64, 3, 0, 1),
 ]
 
 
@@ -180,8 +180,8 @@ def autotune(batch: Batch, candidates=None, steps: int = 8, cache: str | None = 
     """Times each candidate schedule for `steps` steps (after one untimed
     step), in `rounds` interleaved rounds, and keeps the one with the fastest
     best round (one short sample per candidate let timing noise pick a
-    schedule 9 % slower than the best); where that is a dispatched schedule,
-    then times the size classes (SIZE_CLASSES: smaller instances with their
+    schedule 9 % slower than the best); where that is a dispatched or fused
+    schedule, then times the size classes (SIZE_CLASSES: smaller instances with their
     own tile shapes) on it the same way.  Applies the winner and
     re-initialises the batch.  Results are cached per ensemble signature in
     `cache` (JSON)."""
@@ -210,8 +210,13 @@ def autotune(batch: Batch, candidates=None, steps: int = 8, cache: str | None = 
     cls_dt = {(0, 0, 0): timings[0][0]}
     if best[2] >= 2:
         batch.set_schedule(*best)
+        classes = list(size_classes or SIZE_CLASSES)
+        if best[2] == 3:
+            # the fused step has no x tiles: classes that differ only in x
+            # tile rows are one class
+            classes = list(dict.fromkeys((side, yc, 0) for side, yc, _ in classes))
         for _ in range(max(1, rounds)):
-            for cls in (size_classes or SIZE_CLASSES):
+            for cls in classes:
                 dt = _time(batch, steps, lambda: batch.set_size_class(*cls))
                 cls_dt[cls] = min(dt, cls_dt.get(cls, dt))
                 if log:
