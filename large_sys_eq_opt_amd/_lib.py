@@ -30,7 +30,7 @@ class GridDesc(C.Structure):
 
 class LaunchShape(C.Structure):
     _fields_ = [("seg_x", _I), ("seg_y", _I), ("y_waves", _I), ("x_layout", _I), ("x_wpb", _I), ("y_wpb", _I),
-                ("min_tiles", _I), ("fused", _I)]
+                ("min_tiles", _I), ("fused", _I), ("fused_tail", _I)]
 
 
 class BcDesc(C.Structure):

@@ -452,7 +452,7 @@ int valid_bc(const mh_bc* b) {
 // The shape is process-wide: mh_set_launch_shape replaces it under a lock
 // and every launch reads one consistent copy (shape()), so a host thread
 // that sets it while another launches races on nothing.
-static mh_launch_shape g_shape = {1020, 128, 4, 2, 1, 4, 8192, 1};
+static mh_launch_shape g_shape = {1020, 128, 4, 2, 1, 4, 8192, 1, 16};
 static std::mutex g_shape_lock;
 
 mh_launch_shape shape() {
@@ -1262,6 +1262,7 @@ int mh_set_launch_shape(const mh_launch_shape* s) {
     if (s->min_tiles < 0) return fail(MH_ERR_INVALID, "mh_set_launch_shape: negative min_tiles");
     if (s->y_waves != 1 && s->y_waves != 4) return fail(MH_ERR_INVALID, "mh_set_launch_shape: y_waves must be 1 or 4");
     if (s->fused < 0 || s->fused > 2) return fail(MH_ERR_INVALID, "mh_set_launch_shape: fused must be 0, 1 or 2");
+    if (s->fused_tail < 0) return fail(MH_ERR_INVALID, "mh_set_launch_shape: negative fused_tail");
     if (s->x_layout < 0 || s->x_layout > 2) return fail(MH_ERR_INVALID, "mh_set_launch_shape: x_layout must be 0, 1 or 2");
     if ((s->x_wpb != 1 && s->x_wpb != 4) || (s->y_wpb != 1 && s->y_wpb != 4))
         return fail(MH_ERR_INVALID, "mh_set_launch_shape: wavefronts per workgroup must be 1 or 4");

@@ -240,12 +240,13 @@ def sweep_gpu(Uin, Uout, dt, direction, solver, stream=None):
          _stream(stream))
 
 
-SHAPE_FIELDS = ("seg_x", "seg_y", "y_waves", "x_layout", "x_wpb", "y_wpb", "min_tiles", "fused")
+SHAPE_FIELDS = ("seg_x", "seg_y", "y_waves", "x_layout", "x_wpb", "y_wpb", "min_tiles", "fused", "fused_tail")
 
 
 def set_launch_shape(**kw):
     """Sets fields of the sweeps' launch shape (see mh_launch_shape):
-    seg_x, seg_y, y_waves, x_layout, x_wpb, y_wpb, min_tiles, fused."""
+    seg_x, seg_y, y_waves, x_layout, x_wpb, y_wpb, min_tiles, fused,
+    fused_tail."""
     cur = get_launch_shape()
     cur.update({k: int(v) for k, v in kw.items()})
     shape = _lib.LaunchShape(*(cur[f] for f in SHAPE_FIELDS))

@@ -60,7 +60,8 @@ def test_invalid_arguments_return_status_not_crash(hip):
     assert lib.mh_riemann_flux(-1, None, None, 0, 0, None, None) == _lib.MH_ERR_INVALID
     assert lib.mh_riemann_flux(1, None, None, 0, 9, None, None) == _lib.MH_ERR_INVALID
     for bad_shape in ((-1, 0, 1, 0, 1, 1, 0), (0, 0, 3, 0, 1, 1, 0), (0, 0, 1, 3, 1, 1, 0), (0, 0, 1, 0, 2, 1, 0),
-                      (0, 0, 1, 0, 1, 1, -1), (0, 0, 1, 0, 1, 1, 0, 3), (0, 0, 1, 0, 1, 1, 0, -1)):
+                      (0, 0, 1, 0, 1, 1, -1), (0, 0, 1, 0, 1, 1, 0, 3), (0, 0, 1, 0, 1, 1, 0, -1),
+                      (0, 0, 1, 0, 1, 1, 0, 1, -1)):
         assert lib.mh_set_launch_shape(C.byref(_lib.LaunchShape(*bad_shape))) == _lib.MH_ERR_INVALID
     assert lib.mh_set_launch_shape(None) == _lib.MH_ERR_INVALID
     x = (C.c_double * 4)()

@@ -86,6 +86,24 @@ def test_fused_partial_strips_bitwise(hip, oracle, solver, nx, ny, seg):
 
 
 @pytest.mark.parametrize("solver", BITWISE_SOLVERS)
+@pytest.mark.parametrize("nx,ny,seg,min_tiles,tail", [(192, 150, 37, 8, 4), (320, 257, 64, 40, 16), (130, 300, 0, 8, 7),
+                                                      (64, 96, 48, 2, 1), (200, 64, 16, 16, 16)])
+def test_fused_tail_chunks_bitwise(hip, oracle, solver, nx, ny, seg, min_tiles, tail):
+    """The last rows of every strip in short chunks dispatched last (the
+    launch shape's fused_tail; min_tiles sets how many): ragged big and tail
+    chunks, a tail of one-row chunks, and a tail as long as the big chunks
+    (none taken); nine steps of the two-blob field."""
+    from large_sys_eq_opt_amd import cases
+
+    steps = 9
+    cfg = cases.CaseConfig(nx, ny, 2, 0.0, 1.0, 0.0, 1.0, 0.8, 0.0)
+    U0 = scaled_field(oracle, nx, ny, 13)
+    o = oracle.run(U0, cfg.dx, cfg.dy, solver, cfg.cfl, max_steps=steps)
+    n, dts, out = _run(hip, cfg, solver, U0, steps, dict(fused=2, seg_y=seg, min_tiles=min_tiles, fused_tail=tail))
+    _check(o, n, dts, out, steps)
+
+
+@pytest.mark.parametrize("solver", BITWISE_SOLVERS)
 def test_fused_blast_bitwise(hip, oracle, solver):
     """The cylindrical blast on 256x256 ([0,2]^2), 30 steps, default shape."""
     from large_sys_eq_opt_amd import cases

@@ -428,6 +428,30 @@ def test_device_loop_snapshots_match_host_loop(hip, solver):
     assert same_bits(d["U"], h["U"])
 
 
+@pytest.mark.parametrize("solver", [0, 1, 3])
+def test_device_loop_snapshots_fused_step(hip, solver):
+    """The same comparison with every step of the device loop fused (the
+    launch shape's fused = 2; the state alternates between the caller's grid
+    and the stepper's scratch grid, and each snapshot stop has to find it in
+    the caller's grid)."""
+    from large_sys_eq_opt_amd.driver import run_case
+    from large_sys_eq_opt_amd.solver import get_launch_shape, set_launch_shape
+
+    h = run_case("shock_bubble_nd", solver, 1, mode="host", num_snapshots=5, keep_snapshots=True)
+    saved = get_launch_shape()
+    try:
+        set_launch_shape(fused=2)
+        d = run_case("shock_bubble_nd", solver, 1, mode="device", num_snapshots=5, keep_snapshots=True)
+    finally:
+        set_launch_shape(**saved)
+    assert d["steps"] == h["steps"]
+    assert same_bits(d["dt"], h["dt"])
+    assert same_bits(d["snap_t"], h["snap_t"]) and len(d["snap_t"]) == 5
+    for k in range(5):
+        assert same_bits(d["snapshots"][k], h["snapshots"][k]), f"snapshot {k + 1}"
+    assert same_bits(d["U"], h["U"])
+
+
 def test_device_loop_max_steps_past_t_end(hip):
     """max_steps beyond the case's t_end: the device loop stops and clamps at
     t_end exactly as the host loop does (same step count, dt sequence and
