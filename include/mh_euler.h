@@ -135,8 +135,8 @@ int mh_sweep(mh_grid in, mh_grid out, double dt, int dir, int solver, mh_stream 
  * grid.  t_end <= 0: no clamp, steps run unconditionally.  bc NULL =
  * transmissive.  Fills the ghosts of U and reduces its max signal speed.
  * With the launch shape's `fused` set (by default for grids of 2^20 cells
- * or more) and the fused step applicable (HLL, HLLC or FORCE), each step
- * is one pass and the state alternates between U and the scratch grid: it
+ * or more, 2^24 for the exact solver), each step is one pass and the state
+ * alternates between U and the scratch grid: it
  * is in U again after mh_stepper_clock, mh_stepper_finish or
  * mh_stepper_destroy, which is when U may be read. */
 int mh_stepper_create(mh_grid U, int solver, double cfl, double t_end, const mh_bc* bc, int dt_log_capacity,
@@ -179,10 +179,10 @@ typedef struct {
                       tile per workgroup; 0 keeps the shape as set
                       (default 8192) */
     int fused;    /* steppers created from now on run each step as one fused
-                     x-then-y pass where it applies (HLL, HLLC, FORCE),
-                     with seg_y rows per chunk: 1 (default) on grids of at
-                     least 2^20 cells, 2 on every such grid; 0: always the
-                     x sweep and the y sweep */
+                     x-then-y pass, with seg_y rows per chunk: 1 (default)
+                     on grids of at least 2^20 cells (exact solver: 2^24),
+                     2 on every grid; 0: always the x sweep and the y
+                     sweep */
     int fused_tail; /* the fused step's last rows of every strip in chunks
                        of this many rows, dispatched last: about
                        min_tiles / 2 such tiles, at most half the rows, and

@@ -349,10 +349,11 @@ __device__ __forceinline__ bool exact_star(const Prim& VL, const Prim& VR, doubl
     return finite(u_star);
 }
 
+template <bool F = false>
 __device__ __forceinline__ Cons cons_nt(double rho, double un, double ut, double p, int d) {  // riemann.hpp:40-64
     double u, v;
     if (d == kX) { u = un; v = ut; } else { u = ut; v = un; }
-    return {rho, rho * u, rho * v, p / kGm1 + 0.5 * rho * (u * u + v * v)};
+    return {rho, rho * u, rho * v, dvn<F>(p, kGm1) + 0.5 * rho * (u * u + v * v)};
 }
 
 // Sample the solution at xi = 0 on the left (side = 0) or right (side = 1)
@@ -434,20 +435,22 @@ __device__ __forceinline__ Cons exact_sample(const Prim& VK, double aK, double p
 // u* = 0.5 ((u_n + u_n + 0) - 0) (the sum as exact() forms it: +0 for a
 // zero velocity); the sampled state on the side u* picks is (rho, u*, p)
 // on every wave-speed branch (the head and tail speeds coincide, so no
-// fan).  Then the same conserved state, conversion, check and flux.
-template <int D>
+// fan).  Then the same conserved state, conversion, check and flux.  With F
+// (a fast tile) the same operations take the unscaled division, which gives
+// the same bits there (the fast exact solver below); iters is left alone.
+template <int D, bool F = false, bool R = false>
 __device__ __forceinline__ Cons exact_uniform(const Cons& UL, const Prim& VL, const Cons& UR, const Prim& VR,
-                                              double a, int& iters) {
-    iters = 1;
-    const double fd1 = (1.0 / (VL.r * a)) * 1.0;
-    if (fabs(fd1 + fd1) < 1.0e-14) return hll<D>(UL, VL, UR, VR);
+                                              double a, int& iters, double rrL = 0.0, double rrR = 0.0) {
+    if constexpr (!F) iters = 1;
+    const double fd1 = dvn<F>(1.0, VL.r * a) * 1.0;
+    if (fabs(fd1 + fd1) < 1.0e-14) return hll<D, F, R>(UL, VL, UR, VR, rrL, rrR);
     const double unL = un_of<D>(VL), unR = un_of<D>(VR);
     const double u_star = 0.5 * (unL + unR + 0.0 - 0.0);
-    if (!finite(u_star)) return hll<D>(UL, VL, UR, VR);
+    if (!finite(u_star)) return hll<D, F, R>(UL, VL, UR, VR, rrL, rrR);
     const Prim& VK = u_star >= 0.0 ? VL : VR;
-    const Cons Us = cons_nt(VK.r, u_star, ut_of<D>(VK), VK.p, D);
-    const Prim Vs = c2p(Us);
-    if (!phys_riemann(Vs)) return hll<D>(UL, VL, UR, VR);
+    const Cons Us = cons_nt<F>(VK.r, u_star, ut_of<D>(VK), VK.p, D);
+    const Prim Vs = c2p<F>(Us);
+    if (!phys_riemann<F>(Vs)) return hll<D, F, R>(UL, VL, UR, VR, rrL, rrR);
     return pflux<D>(Us, Vs);
 }
 
@@ -470,14 +473,41 @@ __device__ __forceinline__ Cons exact(const Cons& UL, const Prim& VL, const Cons
     return pflux<D>(Us, Vs);
 }
 
-// F selects the unscaled division / square root for HLL, HLLC and FORCE
-// (the exact solver keeps the compiler's expansions and pow).
+// The exact solver on a fast tile (F): exact()'s checks and, where every
+// face of the wavefront joins equal states, its closed form with the
+// unscaled division (the same bits); any other face is left to the tile's
+// IEEE pass: `iters` takes kExactGaveUp, which fails the tile's verdict
+// (solver_gave_up), and the returned flux is not used.  A fast tile's
+// physicality checks, sound speed and closed form are those of exact() bit
+// for bit, so a wavefront that passes here is the one exact() would send
+// to exact_uniform.
+constexpr int kExactGaveUp = 1 << 30;
+template <int D, bool F, bool R>
+__device__ __forceinline__ Cons exact_fast(const Cons& UL, const Prim& VL, const Cons& UR, const Prim& VR, int& iters,
+                                           double rrL, double rrR) {
+    if (!phys_riemann<F>(VL) || !phys_riemann<F>(VR)) return hll<D, F, R>(UL, VL, UR, VR, rrL, rrR);
+    const double aL = sound_r<F, R>(VL, rrL);
+    const bool uniform = VL.r == VR.r && VL.u == VR.u && VL.v == VR.v && VL.p == VR.p && VL.p >= 1.0e-12 &&
+                         VL.p < 1.0e300 && VL.r < 1.0e300 && aL < 1.0e300;
+    if (__all(uniform)) return exact_uniform<D, F, R>(UL, VL, UR, VR, aL, iters, rrL, rrR);
+    iters |= kExactGaveUp;
+    return UL;
+}
+
+// A fast tile's verdict on its Riemann solves: only the fast exact solver
+// writes iters on a fast tile, and only to give up.
+__device__ __forceinline__ bool solver_gave_up(int iters) { return iters != 0; }
+
+// F selects the unscaled division / square root (a fast tile); the exact
+// solver then solves only uniform wavefronts (exact_fast) and leaves the
+// rest, with the compiler's expansions and pow, to the IEEE pass.
 template <int S, int D, bool F = false, bool R = false>
 __device__ __forceinline__ Cons riemann(const Cons& UL, const Prim& VL, const Cons& UR, const Prim& VR, int& iters,
                                         double rrL = 0.0, double rrR = 0.0) {
     if (S == kHLL) return hll<D, F, R>(UL, VL, UR, VR, rrL, rrR);
     if (S == kHLLC) return hllc<D, F, R>(UL, VL, UR, VR, rrL, rrR);
     if (S == kForce) return force<D, F, R>(UL, VL, UR, VR, rrL, rrR);
+    if constexpr (F) return exact_fast<D, F, R>(UL, VL, UR, VR, iters, rrL, rrR);
     return exact<D>(UL, VL, UR, VR, iters);
 }
 

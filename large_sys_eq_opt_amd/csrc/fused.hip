@@ -43,15 +43,20 @@ __device__ __forceinline__ unsigned long long* speed_slot(const FusedArgs& a) {
     return a.clock_speed ? &a.dts.clock->smax[a.dts.clock->slot ^ 1] : a.speed;
 }
 
-template <int S, bool F>
-__device__ __forceinline__ void fused_tile(const View& in, const View& out, const FusedArgs& a, int tile, double dt,
-                                           PairLds& X, int w, int lane, bool& ok, double& m) {
-    const double dtdx = dt / a.dx, dtdy = dt / a.dy;
+// Strip and rows [j0, j1) of a tile.
+__device__ __forceinline__ void tile_span(const FusedArgs& a, int ny, int tile, int& cg, int& j0, int& j1) {
     const bool tail = tile >= a.big_tiles;
     const int t = tail ? tile - a.big_tiles : tile;
-    const int cg = t % a.ngroups;
-    const int j0 = tail ? a.rbig + (t / a.ngroups) * a.seg2 : (t / a.ngroups) * a.seg;
-    const int j1 = min(j0 + (tail ? a.seg2 : a.seg), in.ny);
+    cg = t % a.ngroups;
+    j0 = tail ? a.rbig + (t / a.ngroups) * a.seg2 : (t / a.ngroups) * a.seg;
+    j1 = min(j0 + (tail ? a.seg2 : a.seg), ny);
+}
+
+// The pair over rows [j0, j1) of strip cg.
+template <int S, bool F>
+__device__ __forceinline__ void fused_rows(const View& in, const View& out, const FusedArgs& a, int cg, int j0, int j1,
+                                           double dt, PairLds& X, int w, int lane, bool& ok, double& m) {
+    const double dtdx = dt / a.dx, dtdy = dt / a.dy;
     const int want = (a.speed || a.clock_speed) ? 1 : 0;
     ok = pair_side<S, F>(w, in, out, cg * 64, j0, j1, dtdx, dtdy, a.bc, a.ghosts, want, X, lane, m);
     if (w == 1) ok = ok && X.xok != 0u;
@@ -74,7 +79,9 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
     }
     bool ok = true;
     double m = 0.0;
-    fused_tile<S, true>(in, out, a, tile, dt, X, w, lane, ok, m);
+    int cg, j0, j1;
+    tile_span(a, in.ny, tile, cg, j0, j1);
+    fused_rows<S, true>(in, out, a, cg, j0, j1, dt, X, w, lane, ok, m);
     if (w == 1) {
         if (!ok) {
             if (lane == 0) a.redo[2 + atomicAdd(a.redo, 1)] = tile;
@@ -85,10 +92,16 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
     }
 }
 
-// IEEE pass over the tiles k_step_fused put on the redo list (both
-// wavefronts of a workgroup walk the same tiles, with a barrier between
-// tiles so the ring is free).  The list is emptied by the next step's clock
+// IEEE pass over the tiles k_step_fused put on the redo list.  A list too
+// short to fill the chip splits each tile into pieces of at least kRedoRows
+// rows (with their own warm-up rows) that run side by side, so a few redone
+// tiles cost a few pieces' chains of rows rather than a whole tile's each
+// (the exact solver's redo on the 8192^2 blast, about 250 tiles: 1.20 ms as
+// whole tiles); a long list keeps whole tiles (no extra warm-up rows).  Both
+// wavefronts of a workgroup walk the same pieces, with a barrier between
+// pieces so the ring is free.  The list is emptied by the next step's clock
 // kernel, before the next k_step_fused appends to it.
+constexpr int kRedoRows = 16, kRedoPieces = 2048;
 template <int S>
 __global__ __launch_bounds__(128) void k_step_fused_redo(View g0, View g1, FusedArgs a) {
     __shared__ PairLds X;
@@ -98,22 +111,32 @@ __global__ __launch_bounds__(128) void k_step_fused_redo(View g0, View g1, Fused
     if (n > 0 && dt_of(a.dts, dt)) {
         const bool swap = a.pp && a.dts.clock->cur;
         const View in = swap ? g1 : g0, out = swap ? g0 : g1;
-        for (int q = blockIdx.x; q < n; q += gridDim.x) {
-            const int tile = a.redo[2 + q];
-            bool ok = true;
-            double m = 0.0;
-            fused_tile<S, false>(in, out, a, tile, dt, X, w, lane, ok, m);
-            if (w == 1 && (a.speed || a.clock_speed)) {
-                m = wave_max(m);
-                if (lane == 0) atomic_max_pos(speed_slot(a), m);
+        // pieces per tile, and rows per piece (every piece but a tile's
+        // last has `rows` rows; a tile of fewer rows has fewer pieces)
+        const int longest = max(a.seg, a.seg2);
+        const int pieces = max(1, min((longest + kRedoRows - 1) / kRedoRows, (kRedoPieces + n - 1) / n));
+        const int rows = (longest + pieces - 1) / pieces;
+        for (int q = blockIdx.x; q < n * pieces; q += gridDim.x) {
+            int cg, j0, j1;
+            tile_span(a, in.ny, a.redo[2 + q / pieces], cg, j0, j1);
+            const int p0 = j0 + (q % pieces) * rows;
+            if (p0 < j1) {
+                bool ok = true;
+                double m = 0.0;
+                fused_rows<S, false>(in, out, a, cg, p0, min(p0 + rows, j1), dt, X, w, lane, ok, m);
+                if (w == 1 && (a.speed || a.clock_speed)) {
+                    m = wave_max(m);
+                    if (lane == 0) atomic_max_pos(speed_slot(a), m);
+                }
             }
             pair_barrier();
         }
     }
 }
 
-// The fused step applies with the solvers that have the fast division path.
-bool fused_applies(const mh_grid_s*, int solver) { return solver != kExact; }
+// The fused step applies with every solver (the exact solver's fast pass
+// solves the uniform wavefronts and leaves the rest to the redo pass).
+bool fused_applies(const mh_grid_s*, int) { return true; }
 
 // Rows per chunk of the fused step: the configured y chunk, shortened (to
 // no fewer than 16 rows: each chunk recomputes 4 warm-up rows) while the
@@ -165,10 +188,14 @@ FusedPlan fused_plan(const mh_grid_s* g, const mh_launch_shape& sh) {
 
 int fused_tiles(const mh_grid_s* g) { return fused_plan(g, shape()).ntiles; }
 
+// The redo pass: 256 workgroups walk the list (a few tiles outside the fast
+// range); for the exact solver, whose fast pass leaves every tile with a
+// non-uniform face to it, one workgroup per four tiles.
 template <int S>
 void go_fused(hipStream_t st, const View& v0, const View& v1, const FusedArgs& a) {
     hipLaunchKernelGGL(k_step_fused<S>, dim3((unsigned)a.ntiles), dim3(128), 0, st, v0, v1, a);
-    hipLaunchKernelGGL(k_step_fused_redo<S>, dim3(256), dim3(128), 0, st, v0, v1, a);
+    const unsigned nredo = S == kExact ? (unsigned)std::max(256, (a.ntiles + 3) / 4) : 256u;
+    hipLaunchKernelGGL(k_step_fused_redo<S>, dim3(nredo), dim3(128), 0, st, v0, v1, a);
 }
 
 // One fused step g0 -> g1 (pp = 0), or between g0 and g1 as the clock's cur
@@ -198,7 +225,8 @@ int launch_step_fused(const mh_grid_s* g0, const mh_grid_s* g1, int solver, DtSr
         case kHLL: go_fused<kHLL>(st, v0, v1, a); break;
         case kHLLC: go_fused<kHLLC>(st, v0, v1, a); break;
         case kForce: go_fused<kForce>(st, v0, v1, a); break;
-        default: return fail(MH_ERR_INVALID, "fused step: solver without the fused path");
+        case kExact: go_fused<kExact>(st, v0, v1, a); break;
+        default: return fail(MH_ERR_INVALID, "fused step: unknown solver");
     }
     return check_hip(hipGetLastError(), "fused step launch");
 }

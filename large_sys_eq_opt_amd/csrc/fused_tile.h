@@ -41,12 +41,21 @@ namespace mh {
 
 // One workgroup's LDS (19 KB): eight fit the 160 KB of a CU, so sixteen
 // wavefronts (four per SIMD) can be resident.
+//
+// The exact solver's fast pass (exact_fast) leaves every tile with a
+// non-uniform face to the IEEE pass, so its pair stops at the first row
+// that gives up: the wavefront that finds it sets stop[b & 1] before
+// barrier b, and both read stop[b & 1] after barrier b and leave the tile
+// there, after the same number of barriers.  A flag for barrier b + 1 goes
+// to the other slot, and one for barrier b + 2 can only be written once the
+// reader of slot b & 1 has passed barrier b + 1.
 struct PairLds {
     double ring[4][4][64];   // intermediate rows: slot r & 3, plane, lane
     double M[8][66];         // x exchange: lane l's value at slot l + 1
     double B[12][66];        // window values: left side at slot i, right at 32 + i (row k0 + i)
     unsigned key[64];        // y wavefront: lane's running range key
     unsigned xok;            // x wavefront's fast-range verdict for the tile
+    unsigned stop[2];        // exact solver's fast pass: stop after barrier b if stop[b & 1]
 };
 constexpr int kPairWindow = 32;
 
@@ -98,6 +107,7 @@ __device__ __forceinline__ void pair_x_window(const View& in, int c0, int j0, in
     X.B[7][lane] = right ? Lb.U.e : Fl.e;
     X.B[8][lane] = F ? Lb.rr : Lb.V.r;
     X.B[9][lane] = Lb.V.u; X.B[10][lane] = Lb.V.v; X.B[11][lane] = Lb.V.p;
+    if constexpr (F) key = solver_gave_up(it) ? kRangeKeyEnd : key;
     wave_sync();
 }
 
@@ -114,10 +124,17 @@ __device__ __forceinline__ void pair_x_window(const View& in, int c0, int j0, in
 template <int S, bool F, bool P>
 __device__ __forceinline__ void pair_x(const View& in, int c0, int j0, int j1, double dtd, double half, const Bc& bc,
                                        PairLds& X, int lane) {
+    constexpr bool A = F && S == kExact;   // stops at the first row the solver gives up
     int it = 0;
     unsigned key = 0;
     const int nrows = j1 - j0 + 4;
     const uint32_t lane8 = (uint32_t)(P ? min(c0 + lane, in.nx + 1) - c0 : lane) * (uint32_t)sizeof(double);
+    if constexpr (A) {
+        if (lane == 0) {
+            X.stop[0] = 0u;
+            X.stop[1] = 0u;
+        }
+    }
     auto src_row = [&](int k) { return min(max(j0 - 2 + k, 0), in.ny - 1); };
     Cons Un = ld_off(in, row_at(in, c0, src_row(0)), lane8);
     const double* mL = &X.M[0][lane];
@@ -164,10 +181,16 @@ __device__ __forceinline__ void pair_x(const View& in, int c0, int j0, int j1, d
         double* s = &X.ring[r & 3][0][lane];
         s[0] = Us.r; s[64] = Us.mu; s[128] = Us.mv; s[192] = Us.e;
         if (k == nrows - 1) {
-            const bool ok = F ? !__any(key >= kRangeKeyEnd) : true;
+            const bool ok = F ? !__any(key >= kRangeKeyEnd || solver_gave_up(it)) : true;
             if (lane == 0) X.xok = ok ? 1u : 0u;
         }
+        if constexpr (A) {
+            if (__any(key >= kRangeKeyEnd || solver_gave_up(it)) && lane == 0) X.stop[k & 1] = 1u;
+        }
         pair_barrier();
+        if constexpr (A) {
+            if (X.stop[k & 1]) break;
+        }
     }
 }
 
@@ -179,6 +202,7 @@ __device__ __forceinline__ void pair_x(const View& in, int c0, int j0, int j1, d
 template <int S, bool F, bool P>
 __device__ __forceinline__ bool pair_y(const View& out, int c0, int j0, int j1, double dtd, double half, const Bc& bc,
                                        int ghosts, int speed, PairLds& X, int lane, double& m) {
+    constexpr bool A = F && S == kExact;   // stops at the first row the solver gives up
     int it = 0;
     const int i = c0 + lane;
     // stores: the lane's cell, or the sink column for a lane past the row's end
@@ -191,6 +215,12 @@ __device__ __forceinline__ bool pair_y(const View& out, int c0, int j0, int j1, 
         pair_barrier();
         return held(r);
     };
+    // (A) the pair stops after the barrier of row r; a give-up before the
+    // barrier of row r goes to its slot
+    auto stopped = [&](int r) { return A && X.stop[(r - j0) & 1] != 0u; };
+    auto give_up_before = [&](int r) {
+        if (A && __any(solver_gave_up(it)) && lane == 0) X.stop[(r - j0) & 1] = 1u;
+    };
     unsigned* key = &X.key[lane];
     if constexpr (F) *key = 0u;
     auto conv = [&](const Cons& U, Prim& W) {
@@ -200,21 +230,28 @@ __device__ __forceinline__ bool pair_y(const View& out, int c0, int j0, int j1, 
     m = 0.0;
     Prim dm, W1, W0;
     conv(row(j0 - 2), W0);
+    if (stopped(j0 - 2)) return false;
     conv(row(j0 - 1), W1);
+    if (stopped(j0 - 1)) return false;
     dm = diff(W1, W0);
     Edge L, R, Rp;
     conv(row(j0), W0);
+    if (stopped(j0)) return false;
     dm = reconstruct_d<kY, F>(dm, W1, W0, half, L, Rp);
     W1 = W0;
     conv(row(j0 + 1), W0);
+    if (stopped(j0 + 1)) return false;
     dm = reconstruct_d<kY, F>(dm, W1, W0, half, L, R);
     Cons Fp = riemann<S, kY, F, true>(Rp.U, Rp.V, L.U, L.V, it, Rp.rr, L.rr);
+    give_up_before(j0 + 2);
     Rp = R;
     W1 = W0;
     for (int p = j0 + 2; p <= j1 + 1; ++p) {
         conv(row(p), W0);
+        if (stopped(p)) return false;
         dm = reconstruct_d<kY, F>(dm, W1, W0, half, L, R);
         const Cons Fc = riemann<S, kY, F, true>(Rp.U, Rp.V, L.U, L.V, it, Rp.rr, L.rr);
+        if (p < j1 + 1) give_up_before(p + 1);
         const int r = p - 2;
         const Cons Unew = update(held(r), dtd, Fp, Fc);
         st_off(out, row_at(out, kSinkCol, r), so, Unew);
@@ -239,7 +276,7 @@ __device__ __forceinline__ bool pair_y(const View& out, int c0, int j0, int j1, 
     // their speeds and keys do not count
     const bool active = !P || i < out.nx;
     if (!active) m = 0.0;
-    if constexpr (F) return !__any(active && *key >= kRangeKeyEnd);
+    if constexpr (F) return !__any(active && (*key >= kRangeKeyEnd || solver_gave_up(it)));
     return true;
 }
 

@@ -122,8 +122,13 @@ __global__ __launch_bounds__(64 * WPB) __attribute__((amdgpu_waves_per_eu(W, 8))
     }
 }
 
-// IEEE pass over the tiles the preceding k_sweep_y put on the redo list;
-// the last workgroup to finish empties the list for the next launch.
+// IEEE pass over the tiles the preceding k_sweep_y put on the redo list.  A
+// list too short to fill the chip splits each tile into pieces of at least
+// kRedoRowsY rows (with their own warm-up rows) that run side by side, so a
+// few redone tiles cost a few pieces' chains of rows rather than a whole
+// tile's each; a long list keeps whole tiles.  The last workgroup to finish
+// empties the list for the next launch.
+constexpr int kRedoRowsY = 16, kRedoPiecesY = 4096;
 template <int S>
 __global__ __launch_bounds__(256) void k_sweep_y_redo(View in, View out, SweepArgs a) {
     __shared__ double ring[4][kRing];
@@ -134,11 +139,17 @@ __global__ __launch_bounds__(256) void k_sweep_y_redo(View in, View out, SweepAr
         const double dtd = dt / a.d;
         const double half = 0.5 * dtd;
         const int want = (a.speed || a.clock_speed) ? 1 : 0;
-        for (int q = blockIdx.x * 4 + w; q < n; q += gridDim.x * 4) {
-            const int tile = a.redo[2 + q];
+        // pieces per tile and rows per piece (a tile's last piece may be
+        // shorter, or empty)
+        const int pieces = max(1, min((a.seg + kRedoRowsY - 1) / kRedoRowsY, (kRedoPiecesY + n - 1) / n));
+        const int rows = (a.seg + pieces - 1) / pieces;
+        for (int q = blockIdx.x * 4 + w; q < n * pieces; q += gridDim.x * 4) {
+            const int tile = a.redo[2 + q / pieces];
             const int cg = tile % a.ngroups;
-            const int j0 = (tile / a.ngroups) * a.seg;
-            const int j1 = min(j0 + a.seg, in.ny);
+            const int t0 = (tile / a.ngroups) * a.seg;
+            const int j0 = t0 + (q % pieces) * rows;
+            const int j1 = min(min(j0 + rows, t0 + a.seg), in.ny);
+            if (j0 >= j1) continue;
             double m = 0.0;
             y_tile_pass<S, false>(in, out, cg, j0, j1, dtd, half, a.bc, a.ghosts, want, ring[w], lane, m);
             if (want) {
@@ -466,11 +477,16 @@ void go_x(dim3 g, hipStream_t st, const View& vi, const View& vo, const SweepArg
 }
 // The in-place variant (RL = 0: small grids, the IEEE pass inside the kernel)
 // keeps the compiler's register choice: at the four-wave target it spills.
+// The redo pass: 64 workgroups of four wavefronts walk the list (a few
+// tiles outside the fast range); for the exact solver, whose fast pass
+// leaves every tile with a non-uniform face to it, one wavefront per two
+// tiles.
 template <int S, int W, int WPB>
 void go_y(dim3 g, hipStream_t st, const View& vi, const View& vo, const SweepArgs& a) {
     if (a.redo) {
         hipLaunchKernelGGL((k_sweep_y<S, W, WPB, 1>), g, dim3(64 * WPB), 0, st, vi, vo, a);
-        hipLaunchKernelGGL(k_sweep_y_redo<S>, dim3(64), dim3(256), 0, st, vi, vo, a);
+        const unsigned nredo = S == kExact ? (unsigned)std::max(64, (a.ntiles + 7) / 8) : 64u;
+        hipLaunchKernelGGL(k_sweep_y_redo<S>, dim3(nredo), dim3(256), 0, st, vi, vo, a);
     } else {
         hipLaunchKernelGGL((k_sweep_y<S, 1, WPB, 0>), g, dim3(64 * WPB), 0, st, vi, vo, a);
     }
@@ -923,10 +939,19 @@ int mh_stepper_create(mh_grid U, int solver, double cfl, double t_end, const mh_
     // fused = 1 takes the fused step from kFusedMinCells cells on: smaller
     // grids are latency-bound, and each wavefront pair's chain of rows is
     // longer than the two sweeps' (256^2: 57 against 38 us per step; the
-    // fused step is ahead from 1024^2 on: 73 against 81 us)
-    constexpr int64_t kFusedMinCells = 1 << 20;
+    // fused step is ahead from 1024^2 on: 73 against 81 us).  The exact
+    // solver's fast pass solves the uniform wavefronts and its redo pass
+    // (a pair of 217 VGPRs, two wavefronts per SIMD) every tile with a
+    // non-uniform face; on the cylindrical blast that is ahead of the two
+    // sweeps from 4096^2 on (0.749 against 0.791 ms per step; 8192^2: 2.00
+    // against 2.46; 16384^2: 6.71 against 8.79) and behind below (2048^2:
+    // 0.404 against 0.342), so it starts at 2^24 cells.  On a state with no
+    // uniform region every tile is redone, and the two sweeps stay ahead
+    // (8192^2: 7.93 against 6.90 ms).
+    constexpr int64_t kFusedMinCells = 1 << 20, kFusedMinCellsExact = int64_t(1) << 24;
     const int want = shape().fused;
-    s->fused = (want == 2 || (want == 1 && (int64_t)U->d.nx * U->d.ny >= kFusedMinCells)) &&
+    const int64_t min_cells = solver == kExact ? kFusedMinCellsExact : kFusedMinCells;
+    s->fused = (want == 2 || (want == 1 && (int64_t)U->d.nx * U->d.ny >= min_cells)) &&
                fused_applies(U, solver) && fused_tiles(U) <= s->tmp->redo_cap;
     s->dirty = 0;
     rc = check_hip(hipMalloc((void**)&s->clock, sizeof(Clock)), "hipMalloc(clock)");

@@ -96,6 +96,15 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// The exact solver's fast pass gives up on the tile at the first row or
+// pass with a face it leaves to the IEEE pass (solver_gave_up), which
+// redoes the whole tile: what the fast pass would compute after it is
+// wasted.
+template <int S, bool F> __device__ __forceinline__ bool give_up_now(int it) {
+    if constexpr (F && S == kExact) return __any(solver_gave_up(it));
+    return false;
+}
+
 __device__ __forceinline__ double wave_max(double m) {
     for (int off = 32; off >= 1; off >>= 1) m = smax(m, __shfl_xor(m, off, 64));
     return m;
@@ -217,6 +226,7 @@ __device__ __forceinline__ bool y_tile_pass(const View& in, const View& out, int
         put(0, U0);   // row j0-2 is no longer needed
         dm = reconstruct_d<kY, F>(dm, W1, W0, half, L, R);
         Fp = riemann<S, kY, F, B>(Rp.U, Rp.V, L.U, L.V, it, Rp.rr, L.rr);
+        if (give_up_now<S, F>(it)) return false;
         Rp = R;
         W1 = W0;
     }
@@ -230,6 +240,7 @@ __device__ __forceinline__ bool y_tile_pass(const View& in, const View& out, int
         Un = load_row(min(p + 1, j1 + 1));
         dm = reconstruct_d<kY, F>(dm, W1, W0, half, L, R);
         const Cons Fc = riemann<S, kY, F, B>(Rp.U, Rp.V, L.U, L.V, it, Rp.rr, L.rr);
+        if (give_up_now<S, F>(it)) return false;
         const int r = p - 2;
         const Cons Unew = update(get(slot_old), dtd, Fp, Fc);
         if constexpr (B) st_off(out, row_at(out, kSinkCol, r), so, Unew);
@@ -262,7 +273,8 @@ __device__ __forceinline__ bool y_tile_pass(const View& in, const View& out, int
         slot_new = slot_new == 2 ? 0 : slot_new + 1;
     }
     if constexpr (F)
-        return !__any(__hip_atomic_load(key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT) >= kRangeKeyEnd);
+        return !__any(__hip_atomic_load(key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT) >= kRangeKeyEnd ||
+                      solver_gave_up(it));
     return true;
 }
 
@@ -325,6 +337,10 @@ __device__ __forceinline__ bool x_tile66_pass(const View& in, const View& out, i
             X.R[4][1] = R.V.r; X.R[5][1] = R.V.u; X.R[6][1] = R.V.v; X.R[7][1] = R.V.p;
         }
         const Cons Fc = riemann<S, kX, F>(RU, RV, L.U, L.V, it);   // face q-1/2 = r+1/2, r = p-2
+        if (give_up_now<S, F>(it)) {
+            wave_sync();
+            return false;
+        }
         X.F[0][2 + lane] = Fc.r; X.F[1][2 + lane] = Fc.mu; X.F[2][2 + lane] = Fc.mv; X.F[3][2 + lane] = Fc.e;
         wave_sync();
         const Cons Fm = {X.F[0][1 + lane], X.F[1][1 + lane], X.F[2][1 + lane], X.F[3][1 + lane]};
@@ -352,7 +368,7 @@ __device__ __forceinline__ bool x_tile66_pass(const View& in, const View& out, i
             }
         }
     }
-    if constexpr (F) return !__any(key >= kRangeKeyEnd);
+    if constexpr (F) return !__any(key >= kRangeKeyEnd || solver_gave_up(it));
     return true;
 }
 
@@ -401,6 +417,10 @@ __device__ __forceinline__ bool x_tilec_pass(const View& in, const View& out, in
         const Cons RU = {r1[0], r1[1], r1[2], r1[3]};
         const Prim RV = {r1[4], r1[5], r1[6], r1[7]};
         const Cons Fc = riemann<S, kX, F>(RU, RV, L.U, L.V, it);
+        if (give_up_now<S, F>(it)) {
+            wave_sync();
+            return false;
+        }
         X.M[0][lane] = Fc.r; X.M[1][lane] = Fc.mu; X.M[2][lane] = Fc.mv; X.M[3][lane] = Fc.e;
         wave_sync();
         double f1[4], u2[4];
@@ -433,7 +453,7 @@ __device__ __forceinline__ bool x_tilec_pass(const View& in, const View& out, in
             }
         }
     }
-    if constexpr (F) return !__any(key >= kRangeKeyEnd);
+    if constexpr (F) return !__any(key >= kRangeKeyEnd || solver_gave_up(it));
     return true;
 }
 
@@ -549,6 +569,10 @@ __device__ __forceinline__ bool x_tiled_pass(const View& in, const View& out, in
         const Cons RU = {r1[0], r1[1], r1[2], r1[3]};
         const Prim RV = {r1[0], r1[5], r1[6], r1[7]};
         Fprev = riemann<S, kX, F, B>(RU, RV, L.U, L.V, it, r1[4], L.rr);
+        if (give_up_now<S, F>(it)) {
+            wave_sync();
+            return false;
+        }
         X.M[4][lane] = Fprev.r; X.M[5][lane] = Fprev.mu; X.M[6][lane] = Fprev.mv; X.M[7][lane] = Fprev.e;
     }
     wave_sync();
@@ -556,7 +580,7 @@ __device__ __forceinline__ bool x_tiled_pass(const View& in, const View& out, in
     read_prev(Fm, U2);
     update_prev(niter - 1, Fm, U2);
     wave_sync();   // the slice is free for the next tile of this wavefront
-    if constexpr (F) return !__any(key >= kRangeKeyEnd);
+    if constexpr (F) return !__any(key >= kRangeKeyEnd || solver_gave_up(it));
     return true;
 }
 
@@ -708,6 +732,10 @@ __device__ __forceinline__ bool x_stream_pass(XStream& a, double half, XLdsC& X,
         const Cons RU = {r1[0], r1[1], r1[2], r1[3]};
         const Prim RV = {r1[0], r1[5], r1[6], r1[7]};
         Fprev = riemann<S, kX, F>(RU, RV, L.U, L.V, it);
+        if (give_up_now<S, F>(it)) {
+            wave_sync();
+            return false;
+        }
         X.M[4][lane] = Fprev.r; X.M[5][lane] = Fprev.mu; X.M[6][lane] = Fprev.mv; X.M[7][lane] = Fprev.e;
     }
     wave_sync();
@@ -715,7 +743,7 @@ __device__ __forceinline__ bool x_stream_pass(XStream& a, double half, XLdsC& X,
     read_prev(Fm, U2);
     a.store(niter - 1, U2, Fm, Fprev);
     wave_sync();   // the slice is free for the next tile of this wavefront
-    if constexpr (F) return !__any(key >= kRangeKeyEnd);
+    if constexpr (F) return !__any(key >= kRangeKeyEnd || solver_gave_up(it));
     return true;
 }
 

@@ -17,7 +17,7 @@ from __future__ import annotations
 import numpy as np
 import pytest
 
-from conftest import interior, same_bits
+from conftest import interior, rel_linf, same_bits
 from test_gpu_fast_range import scaled_field, tiny_momentum_field
 
 pytestmark = pytest.mark.gpu
@@ -213,3 +213,141 @@ def test_fused_equals_sweeps_8192(hip):
         set_launch_shape(**saved)
     assert same_bits(res[0][0], res[1][0])
     assert same_bits(res[0][1], res[1][1])
+
+
+EXACT = 2
+
+
+def _exact_case(name, nx, ny):
+    from large_sys_eq_opt_amd import cases
+
+    cfg = cases.get_case_config(name)
+    kind, params = cases.case_params(name)
+    c = cases.CaseConfig(nx, ny, 2, cfg.x_min, cfg.x_max, cfg.y_min, cfg.y_max, cfg.cfl, 0.0)
+    return c, kind, params, cases.case_boundary(name)
+
+
+@pytest.mark.parametrize("case, nx, ny, steps, shape", [
+    ("cyl_blast", 256, 256, 30, dict(seg_y=0, min_tiles=8192)),
+    ("cyl_blast", 200, 131, 20, dict(seg_y=37, min_tiles=16, fused_tail=5)),
+    ("shock_bubble_nd", 256, 128, 25, dict(seg_y=48, min_tiles=0)),
+    ("shock_bubble_nd", 130, 90, 25, dict(seg_y=0, min_tiles=8, fused_tail=4)),
+])
+def test_fused_exact_equals_sweeps(hip, oracle, case, nx, ny, steps, shape):
+    """The exact solver's fused step: its fast pass solves the wavefronts
+    whose faces all join equal states (the closed form) and leaves every
+    tile with another face to the redo pass (the full solver).  The state
+    and dt sequence equal the two-sweep steps (whose y sweep takes the same
+    fast pass and redo list) bit for bit, and the oracle to the exact
+    solver's tolerance (1e-10 relative state, 1e-12 relative dt: the device
+    pow); ragged chunks, tail chunks, partial strips and inflow."""
+    cfg, kind, params, bc = _exact_case(case, nx, ny)
+    U0 = oracle.init(kind, params.array(), nx, ny, cfg.bounds)
+    inflow = bc.left_inflow_state if case == "shock_bubble_nd" else None
+    o = oracle.run(U0, cfg.dx, cfg.dy, EXACT, cfg.cfl, max_steps=steps, inflow_left=inflow)
+    n1, d1, u1 = _run(hip, cfg, EXACT, U0, steps, dict(fused=2, **shape), bc=bc)
+    n0, d0, u0 = _run(hip, cfg, EXACT, U0, steps, dict(fused=0, **shape), bc=bc)
+    assert n1 == n0 == steps
+    assert same_bits(d1, d0)
+    assert same_bits(interior(u1), interior(u0))
+    assert np.max(np.abs(d1 - o["dt"]) / o["dt"]) <= 1e-12
+    assert rel_linf(u1, o["U"]) <= 1e-10
+
+
+def test_fused_exact_mixed_range(hip, oracle):
+    """The two-blob field (part of it outside the fast range, where the
+    whole tile goes to the redo pass) under the exact solver: fused = two
+    sweeps bit for bit, oracle to 1e-10."""
+    from large_sys_eq_opt_amd import cases
+
+    nx, ny, steps = 192, 150, 9
+    cfg = cases.CaseConfig(nx, ny, 2, 0.0, 1.0, 0.0, 1.0, 0.8, 0.0)
+    U0 = scaled_field(oracle, nx, ny, 7)
+    o = oracle.run(U0, cfg.dx, cfg.dy, EXACT, cfg.cfl, max_steps=steps)
+    n1, d1, u1 = _run(hip, cfg, EXACT, U0, steps, dict(fused=2, seg_y=37, min_tiles=0))
+    n0, d0, u0 = _run(hip, cfg, EXACT, U0, steps, dict(fused=0, seg_y=37, min_tiles=0))
+    assert n1 == n0 == steps
+    assert same_bits(d1, d0) and same_bits(interior(u1), interior(u0))
+    assert rel_linf(u1, o["U"]) <= 1e-10
+
+
+def test_fused_exact_8192_equals_sweeps(hip):
+    """Two exact steps of the 8192^2 blast (the benchmark grid; fused by
+    default there): fused and two-sweep steppers agree bit for bit."""
+    from large_sys_eq_opt_amd import Grid2DGPU, Stepper, cases
+    from large_sys_eq_opt_amd.solver import get_launch_shape, set_launch_shape
+
+    n, steps = 8192, 2
+    cfg = cases.get_n_case_config("cyl_blast", n // 256)
+    kind, params = cases.case_params("cyl_blast")
+    res = []
+    saved = get_launch_shape()
+    try:
+        for fused in (1, 0):
+            set_launch_shape(fused=fused)
+            g = Grid2DGPU.from_config(cfg)
+            g.init_state(kind, params.array())
+            st = Stepper(g, EXACT, cfg.cfl, 0.0, cases.case_boundary("cyl_blast"), log_capacity=steps)
+            st.advance(steps)
+            st.finish()
+            _, k, dts = st.clock()
+            st.close()
+            assert k == steps
+            res.append((dts, g.download_to_aos()[::7, ::5]))
+            g.close()
+    finally:
+        set_launch_shape(**saved)
+    assert same_bits(res[0][0], res[1][0])
+    assert same_bits(res[0][1], res[1][1])
+
+
+def _smooth_state(nx, ny):
+    """A ghosted AoS state with no two equal neighbours (smooth density,
+    velocity and pressure waves): no wavefront of the exact solver's fast
+    pass is uniform."""
+    y, x = np.mgrid[0:ny + 4, 0:nx + 4] + 0.5
+    x, y = x / (nx + 4), y / (ny + 4)
+    rho = 1.0 + 0.3 * np.sin(2 * np.pi * x + 0.3) * np.cos(2 * np.pi * y + 1.1)
+    u = 0.3 * np.sin(2 * np.pi * (y + 0.37 * x) + 0.7)
+    v = 0.2 * np.cos(2 * np.pi * (x - 0.41 * y) + 2.0)
+    p = 1.0 + 0.4 * np.cos(2 * np.pi * (x + 0.83 * y) + 0.4)
+    U = np.empty((ny + 4, nx + 4, 4))
+    U[..., 0], U[..., 1], U[..., 2] = rho, rho * u, rho * v
+    U[..., 3] = p / 0.4 + 0.5 * rho * (u * u + v * v)
+    return U
+
+
+def _cold_state(oracle, nx, ny, seed):
+    """A cold, nearly still gas below the fast range (every tile redone)."""
+    rng = np.random.default_rng(seed)
+    V = np.empty((ny + 4, nx + 4, 4))
+    V[..., 0] = 1.0 + 0.2 * rng.random((ny + 4, nx + 4))
+    V[..., 1] = 1e-8 * rng.standard_normal((ny + 4, nx + 4))
+    V[..., 2] = 1e-8 * rng.standard_normal((ny + 4, nx + 4))
+    V[..., 3] = 2e-13 * (1.0 + 0.1 * rng.random((ny + 4, nx + 4)))
+    return oracle.prim_to_cons(V.reshape(-1, 4)).reshape(ny + 4, nx + 4, 4)
+
+
+@pytest.mark.parametrize("solver", [1, EXACT])
+def test_many_tiles_redone(hip, oracle, solver):
+    """A redo list long enough that each redone tile is split into fewer
+    pieces than its rows allow (2560x2048: 640 fused tiles of 128 rows, 640
+    y tiles of 128 rows with the redo list): every tile goes to the redo
+    pass (HLLC: a cold gas below the fast range; exact: a state with no
+    uniform wavefront), and every row of every tile must be redone.  Three
+    steps, fused and two sweeps, against the oracle."""
+    from large_sys_eq_opt_amd import cases
+
+    nx, ny, steps = 2560, 2048, 3
+    cfg = cases.CaseConfig(nx, ny, 2, 0.0, 1.0, 0.0, 1.0, 0.8, 0.0)
+    U0 = _cold_state(oracle, nx, ny, 9) if solver == 1 else _smooth_state(nx, ny)
+    o = oracle.run(U0, cfg.dx, cfg.dy, solver, cfg.cfl, max_steps=steps)
+    for shape in (dict(fused=2, seg_y=128), dict(fused=0, seg_y=128, min_tiles=0)):
+        n, dts, out = _run(hip, cfg, solver, U0, steps, shape)
+        assert n == steps, shape
+        if solver == EXACT:
+            assert np.max(np.abs(dts - o["dt"]) / o["dt"]) <= 1e-12, shape
+            assert rel_linf(out, o["U"]) <= 1e-10, shape
+        else:
+            assert same_bits(dts, o["dt"]), shape
+            assert same_bits(interior(out), interior(o["U"])), shape
