@@ -136,9 +136,12 @@ int mh_sweep(mh_grid in, mh_grid out, double dt, int dir, int solver, mh_stream 
  * transmissive.  Fills the ghosts of U and reduces its max signal speed.
  * With the launch shape's `fused` set (by default for grids of 2^20 cells
  * or more, 2^24 for the exact solver), each step is one pass and the state
- * alternates between U and the scratch grid: it
- * is in U again after mh_stepper_clock, mh_stepper_finish or
- * mh_stepper_destroy, which is when U may be read. */
+ * alternates between U and the scratch grid: it is in U again after
+ * mh_stepper_clock, mh_stepper_finish or mh_stepper_destroy, which is when
+ * U may be read.  An exact-solver stepper under the default shape changes
+ * between the fused step and the two sweeps at its clock reads, by the
+ * share of tiles its fast passes left to the IEEE redo pass (the same
+ * results either way). */
 int mh_stepper_create(mh_grid U, int solver, double cfl, double t_end, const mh_bc* bc, int dt_log_capacity,
                       mh_stepper* out, mh_stream stream);
 int mh_stepper_destroy(mh_stepper s);

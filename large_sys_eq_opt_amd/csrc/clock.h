@@ -12,11 +12,15 @@ namespace mh {
 // sets done and stopped, and mh_stepper_set_stop releases it.
 // With pp set (a stepper on the fused step), committed steps alternate the
 // state between the stepper's grid (cur = 0) and its second grid (cur = 1).
+// redo_tiles counts the tiles the steps' fast passes left to their IEEE
+// redo lists (the fused step's, or the y sweep's), for the stepper's choice
+// between the two.
 struct Clock {
     double t, dt, t_end, cfl, h, t_stop;
     unsigned long long smax[2];   // bit patterns of non-negative doubles
     int step, slot, pending, done, status, log_cap, stopped;
     int pp, cur;
+    int redo_tiles;
 };
 
 // dt source of a sweep: a host value or the device clock.

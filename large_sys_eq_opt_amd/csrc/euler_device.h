@@ -26,6 +26,10 @@ constexpr double kGm1 = kGamma - 1.0;
 constexpr double kFloor = 1.0e-12;      // src/cpu/solver_cpu.cpp:19-20
 
 enum : int { kHLL = 0, kHLLC = 1, kExact = 2, kForce = 3 };
+// Template argument only: the exact solver with the full solve on fast tiles
+// too (exact() with the fast tile's conversions and reconstruction), for
+// states where few wavefronts are uniform.
+constexpr int kExactFull = 4;
 enum : int { kX = 0, kY = 1 };
 
 struct Cons { double r, mu, mv, e; };   // rho, rho*u, rho*v, E
@@ -494,9 +498,10 @@ __device__ __forceinline__ Cons exact_fast(const Cons& UL, const Prim& VL, const
     return UL;
 }
 
-// A fast tile's verdict on its Riemann solves: only the fast exact solver
-// writes iters on a fast tile, and only to give up.
-__device__ __forceinline__ bool solver_gave_up(int iters) { return iters != 0; }
+// A fast tile's verdict on its Riemann solves: whether the fast exact
+// solver gave up on a face (the full exact solver, which kExactFull runs on
+// fast tiles, leaves its iteration count there, below that bit).
+__device__ __forceinline__ bool solver_gave_up(int iters) { return (iters & kExactGaveUp) != 0; }
 
 // F selects the unscaled division / square root (a fast tile); the exact
 // solver then solves only uniform wavefronts (exact_fast) and leaves the
@@ -507,7 +512,7 @@ __device__ __forceinline__ Cons riemann(const Cons& UL, const Prim& VL, const Co
     if (S == kHLL) return hll<D, F, R>(UL, VL, UR, VR, rrL, rrR);
     if (S == kHLLC) return hllc<D, F, R>(UL, VL, UR, VR, rrL, rrR);
     if (S == kForce) return force<D, F, R>(UL, VL, UR, VR, rrL, rrR);
-    if constexpr (F) return exact_fast<D, F, R>(UL, VL, UR, VR, iters, rrL, rrR);
+    if constexpr (F && S == kExact) return exact_fast<D, F, R>(UL, VL, UR, VR, iters, rrL, rrR);
     return exact<D>(UL, VL, UR, VR, iters);
 }
 

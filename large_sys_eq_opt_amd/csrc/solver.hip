@@ -160,6 +160,7 @@ __global__ __launch_bounds__(256) void k_sweep_y_redo(View in, View out, SweepAr
     }
     __syncthreads();
     if (threadIdx.x == 0 && atomicAdd(a.redo + 1, 1) == (int)gridDim.x - 1) {
+        if (a.dts.clock) a.dts.clock->redo_tiles += n;
         a.redo[0] = 0;
         a.redo[1] = 0;
     }
@@ -232,7 +233,10 @@ __global__ void k_zero_u64(unsigned long long* p, int n) {
 // `redo` (a fused stepper's redo list, or null) is emptied for the step.
 __global__ void k_clock(Clock* c, double* dt_log, int commit_only, int* redo) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    if (redo) redo[0] = 0;
+    if (redo) {
+        c->redo_tiles += redo[0];
+        redo[0] = 0;
+    }
     if (c->pending) {
         c->t += c->dt;
         c->step += 1;
@@ -284,11 +288,21 @@ __global__ void k_clock_init(Clock* c, double t_end, double cfl, double h, int l
     c->status = 0;
     c->log_cap = log_cap;
     c->stopped = 0;
+    c->redo_tiles = 0;
 }
 
 // The host has swapped the stepper's two grids: the state is in the first.
 __global__ void k_clock_cur0(Clock* c) {
     if (threadIdx.x == 0 && blockIdx.x == 0) c->cur = 0;
+}
+
+// The stepper changes between the fused step (pp = 1) and the two sweeps;
+// the state is in its grid.
+__global__ void k_clock_pp(Clock* c, int pp) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        c->pp = pp;
+        c->cur = 0;
+    }
 }
 
 // Moves the clock's stop time; releases a clock held at the previous one.
@@ -499,7 +513,7 @@ void go_x_s(int layout, int wpb, dim3 g, hipStream_t st, const View& vi, const V
 }
 template <int S>
 void go_y_s(int waves, int wpb, dim3 g, hipStream_t st, const View& vi, const View& vo, const SweepArgs& a) {
-    if (S == kExact || waves != 4) (wpb == 4 ? go_y<S, 1, 4> : go_y<S, 1, 1>)(g, st, vi, vo, a);
+    if (S == kExact || S == kExactFull || waves != 4) (wpb == 4 ? go_y<S, 1, 4> : go_y<S, 1, 1>)(g, st, vi, vo, a);
     else (wpb == 4 ? go_y<S, 4, 4> : go_y<S, 4, 1>)(g, st, vi, vo, a);
 }
 
@@ -533,8 +547,12 @@ int seg_y_for(const mh_grid_s* g, const mh_launch_shape& sh) {
     return seg;
 }
 
+// full: the exact solver's fast pass solves every face with the full
+// solver (kExactFull) instead of leaving the tiles with a non-uniform
+// wavefront to the IEEE pass (an exact-solver stepper whose fast passes
+// left most tiles to it).
 int launch_sweep(const mh_grid_s* in, const mh_grid_s* out, int dir, int solver, DtSrc dts, const Bc& bc, int ghosts,
-                 unsigned long long* speed, int clock_speed, hipStream_t st) {
+                 unsigned long long* speed, int clock_speed, hipStream_t st, int full = 0) {
     SweepArgs a;
     a.dts = dts;
     a.bc = bc;
@@ -555,7 +573,7 @@ int launch_sweep(const mh_grid_s* in, const mh_grid_s* out, int dir, int solver,
         switch (solver) {
             case kHLL: go_x_s<kHLL>(L, wpb, g, st, vi, vo, a); break;
             case kHLLC: go_x_s<kHLLC>(L, wpb, g, st, vi, vo, a); break;
-            case kExact: go_x_s<kExact>(L, wpb, g, st, vi, vo, a); break;
+            case kExact: (full ? go_x_s<kExactFull> : go_x_s<kExact>)(L, wpb, g, st, vi, vo, a); break;
             default: go_x_s<kForce>(L, wpb, g, st, vi, vo, a); break;
         }
     } else {
@@ -573,7 +591,7 @@ int launch_sweep(const mh_grid_s* in, const mh_grid_s* out, int dir, int solver,
         switch (solver) {
             case kHLL: go_y_s<kHLL>(W, wpb, g, st, vi, vo, a); break;
             case kHLLC: go_y_s<kHLLC>(W, wpb, g, st, vi, vo, a); break;
-            case kExact: go_y_s<kExact>(W, wpb, g, st, vi, vo, a); break;
+            case kExact: (full ? go_y_s<kExactFull> : go_y_s<kExact>)(W, wpb, g, st, vi, vo, a); break;
             default: go_y_s<kForce>(W, wpb, g, st, vi, vo, a); break;
         }
     }
@@ -637,6 +655,12 @@ struct mh_stepper_s {
     // tmp (the clock's cur); dirty: steps were enqueued since the host last
     // moved the state back into U (stepper_settle)
     int fused, dirty;
+    // adaptive (the exact solver under the default shape): at every clock
+    // read the stepper takes the fused step or the two sweeps by the share
+    // of tiles the fast passes left to the redo lists since the last read
+    // (from mark_step, mark_redo); full: its two sweeps solve every face in
+    // the fast pass (launch_sweep)
+    int adaptive, mark_step, mark_redo, full;
 };
 
 namespace mh {
@@ -645,6 +669,8 @@ namespace mh {
 // Enqueues only.  Used by mh_stepper_create and the adjoint's forward run.
 int stepper_start(mh_stepper s, double cfl, hipStream_t st) {
     s->cfl = cfl;
+    s->mark_step = 0;
+    s->mark_redo = 0;
     hipLaunchKernelGGL(k_clock_init, dim3(1), dim3(64), 0, st, s->clock, s->t_end, cfl,
                        std::min(s->U->dx, s->U->dy), s->log_cap, s->fused);
     s->dirty = 0;
@@ -660,10 +686,10 @@ int stepper_step(mh_stepper s, hipStream_t st) {
     const DtSrc dts{s->clock, 0.0};
     s->dirty = 1;
     if (s->fused) return launch_step_fused(s->U, s->tmp, s->solver, dts, s->bc, 1, nullptr, 1, 1, s->tmp->redo, st);
-    int rc = launch_sweep(s->U, s->tmp, kX, s->solver, dts, s->bc, 1, nullptr, 0, st);
+    int rc = launch_sweep(s->U, s->tmp, kX, s->solver, dts, s->bc, 1, nullptr, 0, st, s->full);
     // The y sweep reduces the new state's max signal speed into the
     // clock's inactive slot, which k_clock has just cleared.
-    if (!rc) rc = launch_sweep(s->tmp, s->U, kY, s->solver, dts, s->bc, 1, nullptr, 1, st);
+    if (!rc) rc = launch_sweep(s->tmp, s->U, kY, s->solver, dts, s->bc, 1, nullptr, 1, st, s->full);
     return rc;
 }
 
@@ -678,7 +704,33 @@ int stepper_settle(mh_stepper s, const Clock& c, hipStream_t st) {
     return check_hip(hipGetLastError(), "clock launch");
 }
 
-// Reads the clock (synchronising `st`) and settles the state.
+// The exact solver's fast passes solve the wavefronts whose faces join
+// equal states and leave every tile with another face to an IEEE redo
+// pass: the fused step's (a pair of 217 VGPRs) or the y sweep's.  Where few
+// tiles are left (the cylindrical blast at 8192^2: about 2 % of them) the
+// fused step is faster (2.00 against 2.46 ms per step), where all are (a
+// state with no two equal neighbours) the two sweeps are (6.9 against 7.9
+// ms, with the full solver in their fast pass: no redo pass, whose IEEE
+// division everywhere else costs more).  At a clock read after at least 8
+// steps, a fused stepper that redid more than 40 % of its tiles since the
+// last read changes to the two sweeps with the full solver, and stays
+// there (they leave no redo count to measure the share by).  The state is
+// in the stepper's grid (stepper_settle ran).
+int stepper_adapt(mh_stepper s, const Clock& c, hipStream_t st) {
+    const int steps = c.step - s->mark_step;
+    if (!s->fused || steps < 8) return MH_OK;
+    const double share = (double)(c.redo_tiles - s->mark_redo) / ((double)steps * fused_tiles(s->U));
+    s->mark_step = c.step;
+    s->mark_redo = c.redo_tiles;
+    if (share <= 0.4) return MH_OK;
+    s->fused = 0;
+    s->full = 1;
+    hipLaunchKernelGGL(k_clock_pp, dim3(1), dim3(64), 0, st, s->clock, 0);
+    return check_hip(hipGetLastError(), "clock launch");
+}
+
+// Reads the clock (synchronising `st`), settles the state and, for an
+// adaptive stepper, chooses the step's form for the next steps.
 int stepper_sync(mh_stepper s, hipStream_t st, Clock* out) {
     Clock c;
     hipLaunchKernelGGL(k_clock, dim3(1), dim3(64), 0, st, s->clock, s->dt_log, 1, nullptr);
@@ -686,13 +738,17 @@ int stepper_sync(mh_stepper s, hipStream_t st, Clock* out) {
     if (!rc) rc = check_hip(hipMemcpyAsync(&c, s->clock, sizeof(c), hipMemcpyDeviceToHost, st), "hipMemcpyAsync");
     if (!rc) rc = check_hip(hipStreamSynchronize(st), "hipStreamSynchronize");
     if (!rc) rc = stepper_settle(s, c, st);
+    if (!rc && s->adaptive) rc = stepper_adapt(s, c, st);
     if (!rc && out) *out = c;
     return rc;
 }
 
 // The adjoint's forward run copies checkpoints out of the stepper's grid
 // between steps, so its stepper keeps the two sweeps.
-void stepper_unfuse(mh_stepper s) { s->fused = 0; }
+void stepper_unfuse(mh_stepper s) {
+    s->fused = 0;
+    s->adaptive = 0;
+}
 }  // namespace mh
 
 extern "C" {
@@ -953,6 +1009,7 @@ int mh_stepper_create(mh_grid U, int solver, double cfl, double t_end, const mh_
     const int64_t min_cells = solver == kExact ? kFusedMinCellsExact : kFusedMinCells;
     s->fused = (want == 2 || (want == 1 && (int64_t)U->d.nx * U->d.ny >= min_cells)) &&
                fused_applies(U, solver) && fused_tiles(U) <= s->tmp->redo_cap;
+    s->adaptive = want == 1 && solver == kExact && s->fused;
     s->dirty = 0;
     rc = check_hip(hipMalloc((void**)&s->clock, sizeof(Clock)), "hipMalloc(clock)");
     s->dt_log = nullptr;
@@ -1253,9 +1310,9 @@ int mh_stepper_advance_profiled(mh_stepper s, int nsteps, mh_stream stream, doub
             if (!rc) rc = check_hip(hipEventRecord(e[2], st), "hipEventRecord");
         } else {
             s->dirty = 1;
-            if (!rc) rc = launch_sweep(s->U, s->tmp, kX, s->solver, dts, s->bc, 1, nullptr, 0, st);
+            if (!rc) rc = launch_sweep(s->U, s->tmp, kX, s->solver, dts, s->bc, 1, nullptr, 0, st, s->full);
             if (!rc) rc = check_hip(hipEventRecord(e[2], st), "hipEventRecord");
-            if (!rc) rc = launch_sweep(s->tmp, s->U, kY, s->solver, dts, s->bc, 1, nullptr, 1, st);
+            if (!rc) rc = launch_sweep(s->tmp, s->U, kY, s->solver, dts, s->bc, 1, nullptr, 1, st, s->full);
         }
         if (!rc) rc = check_hip(hipEventRecord(e[3], st), "hipEventRecord");
     }

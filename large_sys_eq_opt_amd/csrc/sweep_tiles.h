@@ -771,15 +771,15 @@ __device__ __forceinline__ void x_tile(const View& in, const View& out, int j, i
                                        int lane) {
     // fast pass first (see y_tile_pass); the IEEE pass redoes the segment if
     // the fast one gives up
+    const bool fast = dt_in_fast_range(dtd);
     if constexpr (L == 1) {
-        if (dt_in_fast_range(dtd) && x_tile66_pass<S, true>(in, out, j, i0, i1, dtd, half, bc, ghosts, X, lane)) return;
+        if (fast && x_tile66_pass<S, true>(in, out, j, i0, i1, dtd, half, bc, ghosts, X, lane)) return;
         x_tile66_pass<S, false>(in, out, j, i0, i1, dtd, half, bc, ghosts, X, lane);
     } else if constexpr (L == 2) {
-        if (dt_in_fast_range(dtd) && x_tiled_pass<S, true, B>(in, out, j, i0, i1, dtd, half, bc, ghosts, X, lane))
-            return;
+        if (fast && x_tiled_pass<S, true, B>(in, out, j, i0, i1, dtd, half, bc, ghosts, X, lane)) return;
         x_tiled_pass<S, false, B>(in, out, j, i0, i1, dtd, half, bc, ghosts, X, lane);
     } else {
-        if (dt_in_fast_range(dtd) && x_tilec_pass<S, true>(in, out, j, i0, i1, dtd, half, bc, ghosts, X, lane)) return;
+        if (fast && x_tilec_pass<S, true>(in, out, j, i0, i1, dtd, half, bc, ghosts, X, lane)) return;
         x_tilec_pass<S, false>(in, out, j, i0, i1, dtd, half, bc, ghosts, X, lane);
     }
 }

@@ -351,3 +351,56 @@ def test_many_tiles_redone(hip, oracle, solver):
         else:
             assert same_bits(dts, o["dt"]), shape
             assert same_bits(interior(out), interior(o["U"])), shape
+
+
+@pytest.mark.parametrize("state", ["smooth", "blast"])
+def test_exact_stepper_changes_form(hip, oracle, state):
+    """Under the default launch shape an exact-solver stepper on 4096^2
+    starts on the fused step and, at a clock read after 10 steps, changes to
+    the two sweeps where its fast passes left most tiles to the redo pass (a
+    state with no uniform wavefront), and stays on the fused step on the
+    cylindrical blast; 12 steps equal a stepper held to the two sweeps bit
+    for bit either way."""
+    from large_sys_eq_opt_amd import Grid2DGPU, Stepper, cases
+    from large_sys_eq_opt_amd.solver import get_launch_shape, set_launch_shape
+
+    n = 4096
+    cfg = cases.get_n_case_config("cyl_blast", n // 256)
+    kind, params = cases.case_params("cyl_blast")
+    bc = cases.case_boundary("cyl_blast")
+
+    def start(g):
+        if state == "smooth":
+            g.upload_from_aos(_smooth_state(n, n))
+        else:
+            g.init_state(kind, params.array())
+
+    saved = get_launch_shape()
+    try:
+        set_launch_shape(fused=1)
+        g = Grid2DGPU.from_config(cfg)
+        start(g)
+        st = Stepper(g, EXACT, cfg.cfl, 0.0, bc, log_capacity=16)
+        st.advance(10)
+        st.clock()
+        _, k = st.advance_profiled(2)
+        _, steps, dts = st.clock()
+        st.close()
+        assert steps == 12
+        assert (k[2] > 0.0) == (state == "smooth"), k   # the y sweep ran: the two sweeps
+        out = g.download_to_aos()[::5, ::3]
+        g.close()
+        set_launch_shape(fused=0)
+        g = Grid2DGPU.from_config(cfg)
+        start(g)
+        st = Stepper(g, EXACT, cfg.cfl, 0.0, bc, log_capacity=16)
+        st.advance(12)
+        _, steps0, dts0 = st.clock()
+        st.close()
+        ref = g.download_to_aos()[::5, ::3]
+        g.close()
+    finally:
+        set_launch_shape(**saved)
+    assert steps0 == 12
+    assert same_bits(dts, dts0)
+    assert same_bits(out, ref)
