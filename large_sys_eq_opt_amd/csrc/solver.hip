@@ -711,14 +711,14 @@ int stepper_settle(mh_stepper s, const Clock& c, hipStream_t st) {
 // fused step is faster (2.00 against 2.46 ms per step), where all are (a
 // state with no two equal neighbours) the two sweeps are (6.9 against 7.9
 // ms, with the full solver in their fast pass: no redo pass, whose IEEE
-// division everywhere else costs more).  At a clock read after at least 8
+// division everywhere else costs more).  At a clock read after at least 4
 // steps, a fused stepper that redid more than 40 % of its tiles since the
 // last read changes to the two sweeps with the full solver, and stays
 // there (they leave no redo count to measure the share by).  The state is
 // in the stepper's grid (stepper_settle ran).
 int stepper_adapt(mh_stepper s, const Clock& c, hipStream_t st) {
     const int steps = c.step - s->mark_step;
-    if (!s->fused || steps < 8) return MH_OK;
+    if (!s->fused || steps < 4) return MH_OK;
     const double share = (double)(c.redo_tiles - s->mark_redo) / ((double)steps * fused_tiles(s->U));
     s->mark_step = c.step;
     s->mark_redo = c.redo_tiles;
