@@ -215,11 +215,11 @@ __device__ __forceinline__ bool pair_y(const View& out, int c0, int j0, int j1, 
         pair_barrier();
         return held(r);
     };
-    // (A) the pair stops after the barrier of row r; a give-up before the
-    // barrier of row r goes to its slot
-    auto stopped = [&](int r) { return A && X.stop[(r - j0) & 1] != 0u; };
+    // (A) the pair stops after the barrier of row r (barrier r - j0 + 2 of
+    // the tile); a give-up before the barrier of row r goes to its slot
+    auto stopped = [&](int r) { return A && X.stop[(r - j0 + 2) & 1] != 0u; };
     auto give_up_before = [&](int r) {
-        if (A && __any(solver_gave_up(it)) && lane == 0) X.stop[(r - j0) & 1] = 1u;
+        if (A && __any(solver_gave_up(it)) && lane == 0) X.stop[(r - j0 + 2) & 1] = 1u;
     };
     unsigned* key = &X.key[lane];
     if constexpr (F) *key = 0u;
