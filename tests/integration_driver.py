@@ -1,10 +1,11 @@
-"""Builds the reference-side C++ driver of INTEGRATION.md: the binding header
-given there (mh_backend.hpp: the reference's Grid2DGPU / compute_dt_gpu /
+"""Builds the reference-side C++ driver of the integration notes (the ```cpp
+blocks of the notes file at the repository root): the binding header given
+there (mh_backend.hpp: the reference's Grid2DGPU / compute_dt_gpu /
 advance_*_gpu API over libmh_euler.so) and a main() with the loop of the
 reference's GPU driver (scripts/gpu/main_gpu.cu:225-299: compute_dt_gpu,
 clamp to t_end, advance_second_order_gpu, swap, t += dt, a snapshot download
 at the first step with t >= k t_end / nsnap), followed by the
-device-resident loop block of INTEGRATION.md from the same initial state.
+device-resident loop block of the notes from the same initial state.
 
 The header includes the reference's own head/types.hpp and head/riemann.hpp,
 so the driver is built where the reference tree is (g++, no HIP compiler)
@@ -116,8 +117,8 @@ int main(int argc, char** argv) {
 
 
 def blocks():
-    """The ```cpp blocks of INTEGRATION.md: the binding header, then the
-    device-resident loop."""
+    """The ```cpp blocks of the integration notes: the binding header, then
+    the device-resident loop."""
     with open(os.path.join(ROOT, "INTEGRATION.md")) as f:
         return re.findall(r"```cpp\n(.*?)```", f.read(), re.S)
 

@@ -1,4 +1,4 @@
-"""The reference-side C++ binding of INTEGRATION.md run on the GPU: the
+"""The reference-side C++ binding of the integration notes run on the GPU: the
 driver tests/integration_driver.py builds (the reference GPU driver's loop,
 scripts/gpu/main_gpu.cu:225-299, over the binding header, then the
 device-resident loop block) runs the cylindrical blast at 256^2 with HLLC to

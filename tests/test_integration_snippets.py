@@ -1,4 +1,4 @@
-"""The C++ code INTEGRATION.md gives a maintainer (the reference's Grid2DGPU /
+"""The C++ code the integration notes give a maintainer (the reference's Grid2DGPU /
 compute_dt_gpu / advance_*_gpu API over libmh_euler.so, and the
 device-resident loop) compiles against include/mh_euler.h and the
 reference's own headers (head/types.hpp, head/riemann.hpp) and links
