@@ -250,8 +250,8 @@ int mh_batch_prepare(mh_batch b, mh_stream stream);
  * (>= 1), y tile chunk rows (0 = whole column), mode (0: one persistent
  * launch per step; 1: two persistent launches, x then y; 2: two dispatched
  * launches with one tile per wavefront; 3: the fused step, a wavefront pair
- * per y tile sweeping x into LDS and y out of it, plus its redo launch --
- * HLL, HLLC, FORCE; the exact solver runs mode 2 instead), persistent
+ * per y tile sweeping x into LDS and y out of it, plus its redo launch),
+ * persistent
  * workgroups per CU (0 = occupancy limit).  Under mode 3 an instance's state
  * alternates between its grid and its scratch grid; mh_batch_sync,
  * mh_batch_clock, mh_batch_set_schedule and mh_batch_set_size_class

@@ -358,17 +358,22 @@ __global__ __launch_bounds__(128) void k_batch_fused_redo(BatchArgs a) {
     const int step = __hip_atomic_load(&a.ctrl->step, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     const int par = step & 1;
     const int n = __hip_atomic_load(a.redo, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    for (int r = blockIdx.x; r < n; r += gridDim.x) {
-        const YTile t = a.yt[a.redo[2 + r]];
+    // a list too short to fill the chip: each tile in up to 8 pieces of at
+    // least 16 rows (each with its own warm-up rows) that run side by side
+    const int pieces = max(1, min(8, (2048 + n - 1) / max(n, 1)));
+    for (int q = blockIdx.x; q < n * pieces; q += gridDim.x) {
+        const YTile t = a.yt[a.redo[2 + q / pieces]];
+        const int rows = max(16, (t.j1 - t.j0 + pieces - 1) / pieces);
+        const int p0 = t.j0 + (q % pieces) * rows;
         const BatchInst& I = a.inst[t.inst];
         const BatchClock& c = a.clk[t.inst];
         double dt;
-        if (inst_dt(I, c, par, dt)) {
+        if (p0 < t.j1 && inst_dt(I, c, par, dt)) {
             const bool swap = __builtin_amdgcn_readfirstlane(c.cur) != 0;
             const View in = uniform(swap ? I.T : I.U), out = uniform(swap ? I.U : I.T);
             const double dtdx = uniform(dt / I.dx), dtdy = uniform(dt / I.dy);
             double m = 0.0;
-            pair_side<S, false>(w, in, out, t.cg * 64, t.j0, t.j1, dtdx, dtdy, I.bc, 1, 1, X, lane, m);
+            pair_side<S, false>(w, in, out, t.cg * 64, p0, min(p0 + rows, t.j1), dtdx, dtdy, I.bc, 1, 1, X, lane, m);
             if (w == 1) {
                 m = wave_max(m);
                 if (lane == 0 && m > 0.0)
@@ -446,7 +451,7 @@ struct mh_batch_s {
     int nxt = 0, nyt = 0;
     // schedule; mode 0: one persistent launch per step, 1: two persistent
     // launches, 2: two dispatched launches (one tile per wavefront), 3: the
-    // fused step (a wavefront pair per y tile; the exact solver takes mode 2)
+    // fused step (a wavefront pair per y tile)
     int xseg = 0, xrows = 1, ychunk = 256, mode = 0, blocks_per_cu = 0;
     // size class: instances with fewer than small_side cells a side take
     // their own y chunk rows and x tile rows (0: the schedule's)
@@ -595,13 +600,19 @@ int launch_step(mh_batch b, hipStream_t st) {
     a.ctrl = b->dctrl;
     a.redo = b->dredo;
     const dim3 g(b->nblocks), blk(256);
-    if (b->mode == 3 && b->solver != kExact) {
-        const dim3 gf((unsigned)std::max(1, b->nyt)), gr((unsigned)std::max(1, std::min(256, b->nyt))), bf(128);
+    if (b->mode == 3) {
+        // the redo pass: 256 workgroups, for the exact solver (whose fast
+        // pass leaves every tile with a non-uniform face to it) one per four
+        // tiles
+        const int nr = b->solver == kExact ? std::max(256, b->nyt / 4) : 256;
+        const dim3 gf((unsigned)std::max(1, b->nyt)), gr((unsigned)std::max(1, std::min(nr, b->nyt))), bf(128);
         switch (b->solver) {
             case kHLL: hipLaunchKernelGGL(k_batch_fused<kHLL>, gf, bf, 0, st, a, b->nyt);
                        hipLaunchKernelGGL(k_batch_fused_redo<kHLL>, gr, bf, 0, st, a); break;
             case kHLLC: hipLaunchKernelGGL(k_batch_fused<kHLLC>, gf, bf, 0, st, a, b->nyt);
                         hipLaunchKernelGGL(k_batch_fused_redo<kHLLC>, gr, bf, 0, st, a); break;
+            case kExact: hipLaunchKernelGGL(k_batch_fused<kExact>, gf, bf, 0, st, a, b->nyt);
+                         hipLaunchKernelGGL(k_batch_fused_redo<kExact>, gr, bf, 0, st, a); break;
             default: hipLaunchKernelGGL(k_batch_fused<kForce>, gf, bf, 0, st, a, b->nyt);
                      hipLaunchKernelGGL(k_batch_fused_redo<kForce>, gr, bf, 0, st, a); break;
         }

@@ -102,7 +102,7 @@ def test_batch_graph_replay_matches_direct_launches(hip, oracle, mode):
 @pytest.mark.parametrize("solver", [0, 2, 3])
 def test_batch_other_solvers(hip, oracle, solver, mode):
     """HLL, exact (within 1e-10: its Newton iteration is not bitwise) and
-    FORCE; under mode 3 the exact solver takes the dispatched launches."""
+    FORCE, under the persistent and the fused schedule."""
     from large_sys_eq_opt_amd.ensemble import Batch
 
     spec = SPEC[:4]
@@ -187,3 +187,29 @@ def test_batch_fused_download_without_clock(hip, oracle):
         o = oracle_instance(oracle, s, 1, 3)
         assert same_bits(interior(b.download(k)), interior(o["U"])), (k, s)
     b.close()
+
+
+def test_batch_exact_fused_equals_dispatched(hip, oracle):
+    """The exact solver under the fused mode (its fast pass solves the
+    uniform wavefronts, the redo pass every tile with another face, in
+    pieces) equals the dispatched two-sweep schedule bit for bit on all
+    eight instances over 20 steps, and its own oracle runs to 1e-10."""
+    from large_sys_eq_opt_amd.ensemble import Batch
+
+    out = []
+    for schedule in [(0, 64, 3, 0, 1), (0, 64, 2, 0, 4)]:
+        b = Batch(SPEC, solver=2)
+        b.set_schedule(*schedule)
+        b.initialise()
+        b.step(20)
+        t, dt, steps, status = b.clock()
+        assert (steps == 20).all() and (status == 0).all(), schedule
+        out.append(([b.download(k) for k in range(len(SPEC))], t, dt))
+        b.close()
+    for k in range(len(SPEC)):
+        assert same_bits(interior(out[0][0][k]), interior(out[1][0][k])), k
+    assert same_bits(out[0][1], out[1][1]) and same_bits(out[0][2], out[1][2])
+    for k in (0, 3, 7):
+        o = oracle_instance(oracle, SPEC[k], 2, 20)
+        U = interior(out[0][0][k])
+        assert np.max(np.abs(U - interior(o["U"]))) <= 1e-10 * np.max(np.abs(interior(o["U"]))), k
