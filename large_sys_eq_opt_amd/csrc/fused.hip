@@ -96,8 +96,9 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
 // short to fill the chip splits each tile into pieces of at least kRedoRows
 // rows (with their own warm-up rows) that run side by side, so a few redone
 // tiles cost a few pieces' chains of rows rather than a whole tile's each
-// (the exact solver's redo on the 8192^2 blast, about 250 tiles: 1.20 ms as
-// whole tiles); a long list keeps whole tiles (no extra warm-up rows).  Both
+// (the exact solver's redo on the 8192^2 blast, about 180 tiles: 1.20 ms as
+// whole tiles, 0.38 ms in pieces); a long list keeps whole tiles (no extra
+// warm-up rows).  Both
 // wavefronts of a workgroup walk the same pieces, with a barrier between
 // pieces so the ring is free.  The list is emptied by the next step's clock
 // kernel, before the next k_step_fused appends to it.
