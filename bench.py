@@ -40,11 +40,12 @@ adjoint: the 512x256 shock-bubble adjoint case (400 HLLC steps, a
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--size N]
        [--solver hllc] [--fused 1|0] [--no-cpu] [--no-extras]
-       [--leg batched|adjoint|snapshot]
+       [--leg batched|adjoint|snapshot|cpu|slab]
 (--leg runs one extra leg alone and prints its JSON object: the profiling
 runs of those workloads use it; "snapshot" measures the cost of the device
 loop's snapshots at --size, "cpu" times the CPU baseline alone at --size
-and --solver.)
+and --solver, "slab" one GPU's y-slab of --size^2 split over 2, 4 and 8
+GPUs.)
 """
 from __future__ import annotations
 
@@ -259,6 +260,43 @@ def adjoint_leg():
             "ratio_vs_checkpointing_forward": b / c, "whole_gradient_ratio_vs_forward": (c + b) / f}
 
 
+def slab_leg(n: int = 8192, steps: int = 200, warmup: int = 30):
+    """One GPU's share of the y-slab decomposition of the n^2 workload over
+    G = 2, 4, 8 GPUs: an n x n/G slab through the blast's centre, HLLC, its
+    top and bottom ghost rows held at a fixed state (a stand-in for the
+    neighbours' halo rows; the exchange itself is not modelled), the device
+    loop timed as the plain leg."""
+    from large_sys_eq_opt_amd import Grid2DGPU, Stepper, cases
+    from large_sys_eq_opt_amd.solver import Boundary, synchronize
+
+    kind, params = cases.case_params("cyl_blast")
+    inner = [1.0, 0.0, 0.0, 1.0 / 0.4]   # the blast's inner state, conserved
+    bc = Boundary((0, 0, 1, 1), [[0.0] * 4, [0.0] * 4, inner, inner])
+    out = []
+    for g in (2, 4, 8):
+        ny = n // g
+        h = 2.0 / n
+        cfg = cases.CaseConfig(n, ny, 2, 0.0, 2.0, 1.0 - 0.5 * ny * h, 1.0 + 0.5 * ny * h, 0.8, 0.0)
+        U = Grid2DGPU.from_config(cfg)
+        U.init_state(kind, params.array())
+        st = Stepper(U, 1, cfg.cfl, 0.0, bc, log_capacity=steps + warmup + 8)
+        st.advance(warmup)
+        st.clock()
+        t0 = time.perf_counter()
+        _, ms_k = st.advance_profiled(steps)
+        wall = time.perf_counter() - t0
+        st.clock()
+        st.close()
+        U.close()
+        synchronize()
+        rate = n * ny * steps / wall
+        out.append({"gpus": g, "slab": f"{n}x{ny}", "per_gpu_Gcell_updates_per_s": rate / 1e9,
+                    "ms_per_step": wall * 1e3 / steps, "kernel_ms": [round(x, 4) for x in ms_k],
+                    "sum_of_slabs_Gcell_updates_per_s": g * rate / 1e9})
+    return {"workload": f"cyl_blast {n}x{n} HLLC split into y-slabs, one slab per GPU, ghost rows held fixed",
+            "slabs": out}
+
+
 def snapshot_leg(n: int = 8192, solver: int = 1, csv_dir: str | None = None):
     """Snapshot cost on the benchmark workload (cyl_blast n^2, run to
     t_end = 0.25 in the device-resident loop): the loop alone; the same loop
@@ -387,7 +425,7 @@ def main():
     ap.add_argument("--solver", default="hllc")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true", help="skip the batched and adjoint legs")
-    ap.add_argument("--leg", choices=["batched", "adjoint", "snapshot", "cpu"], default=None,
+    ap.add_argument("--leg", choices=["batched", "adjoint", "snapshot", "cpu", "slab"], default=None,
                     help="run only this extra leg (batched: against --plain-rate Gcell/s)")
     ap.add_argument("--plain-rate", type=float, default=32.0)
     ap.add_argument("--state", choices=["case", "smooth"], default="case",
@@ -414,6 +452,9 @@ def main():
         return
     if args.leg == "snapshot":
         print(json.dumps(snapshot_leg(args.size)))
+        return
+    if args.leg == "slab":
+        print(json.dumps(slab_leg(args.size)))
         return
     if args.leg == "cpu":
         from large_sys_eq_opt_amd import parse_riemann_solver
