@@ -244,16 +244,22 @@ __device__ __forceinline__ bool pair_y(const View& out, int c0, int j0, int j1, 
     dm = reconstruct_d<kY, F>(dm, W1, W0, half, L, R);
     Cons Fp = riemann<S, kY, F, true>(Rp.U, Rp.V, L.U, L.V, it, Rp.rr, L.rr);
     give_up_before(j0 + 2);
-    Rp = R;
-    W1 = W0;
-    for (int p = j0 + 2; p <= j1 + 1; ++p) {
-        conv(row(p), W0);
+    // Row p of the march: its primitive state Wn, the face below it from the
+    // previous row's right edge state Ro and this row's left edge state, and
+    // the update of row p-2 from the face fluxes Fo (below) and Fn (above).
+    // The carried values (W, R, F) alternate between two sets of registers,
+    // two rows per loop pass, so no copies close the loop (not for the exact
+    // solver, whose wavefront has no registers to spare for the second set:
+    // it copies them after every row).
+    auto step = [&](int p, const Prim& Wo, Prim& Wn, const Edge& Ro, Edge& Rn, const Cons& Fo, Cons& Fn) {
+        conv(row(p), Wn);
         if (stopped(p)) return false;
-        dm = reconstruct_d<kY, F>(dm, W1, W0, half, L, R);
-        const Cons Fc = riemann<S, kY, F, true>(Rp.U, Rp.V, L.U, L.V, it, Rp.rr, L.rr);
+        Edge Lp;
+        dm = reconstruct_d<kY, F>(dm, Wo, Wn, half, Lp, Rn);
+        Fn = riemann<S, kY, F, true>(Ro.U, Ro.V, Lp.U, Lp.V, it, Ro.rr, Lp.rr);
         if (p < j1 + 1) give_up_before(p + 1);
         const int r = p - 2;
-        const Cons Unew = update(held(r), dtd, Fp, Fc);
+        const Cons Unew = update(held(r), dtd, Fo, Fn);
         st_off(out, row_at(out, kSinkCol, r), so, Unew);
         if (speed) m = smax(m, signal_speed<F>(Unew));
         if (ghosts) {
@@ -268,9 +274,21 @@ __device__ __forceinline__ bool pair_y(const View& out, int c0, int j0, int j1, 
                 st(out, out.nx + 1, r, G);
             }
         }
-        Fp = Fc;
-        Rp = R;
-        W1 = W0;
+        return true;
+    };
+    Cons Fq;
+    int p = j0 + 2;
+    if constexpr (S != kExact) {
+        for (; p + 1 <= j1 + 1; p += 2) {
+            if (!step(p, W0, W1, R, Rp, Fp, Fq)) return false;
+            if (!step(p + 1, W1, W0, Rp, R, Fq, Fp)) return false;
+        }
+    }
+    for (; p <= j1 + 1; ++p) {
+        if (!step(p, W0, W1, R, Rp, Fp, Fq)) return false;
+        W0 = W1;
+        R = Rp;
+        Fp = Fq;
     }
     // lanes past a partial strip's last column computed on ghost copies:
     // their speeds and keys do not count
