@@ -136,13 +136,15 @@ __device__ __forceinline__ void pair_x(const View& in, int c0, int j0, int j1, d
         }
     }
     auto src_row = [&](int k) { return min(max(j0 - 2 + k, 0), in.ny - 1); };
-    Cons Un = ld_off(in, row_at(in, c0, src_row(0)), lane8);
+    Cons Ua = ld_off(in, row_at(in, c0, src_row(0)), lane8), Ub;
     const double* mL = &X.M[0][lane];
     const double* mR = &X.M[0][lane + 2];
-    for (int k = 0; k < nrows; ++k) {
+    // Row k from its state U0 (loaded one row ahead), loading row k+1 into
+    // Un; false: the pair stops here (A).  The loaded rows alternate between
+    // two sets of registers, two rows per loop pass, as in the y wavefront.
+    auto step = [&](int k, const Cons& U0, Cons& Un) {
         const int w = k & (kPairWindow - 1);
         if (w == 0) pair_x_window<S, F, P>(in, c0, j0, k, half, X, lane, key);
-        const Cons U0 = Un;
         Un = ld_off(in, row_at(in, c0, src_row(min(k + 1, nrows - 1))), lane8);
         const Prim W0 = c2p<F>(U0);
         if constexpr (F) key = max(key, range_key(U0.r, U0.e, W0.p));
@@ -189,8 +191,21 @@ __device__ __forceinline__ void pair_x(const View& in, int c0, int j0, int j1, d
         }
         pair_barrier();
         if constexpr (A) {
-            if (X.stop[k & 1]) break;
+            if (X.stop[k & 1]) return false;
         }
+        return true;
+    };
+    int k = 0;
+    if constexpr (S != kExact) {
+        // the window pass falls on even rows only (kPairWindow is even)
+        for (; k + 1 < nrows; k += 2) {
+            step(k, Ua, Ub);
+            step(k + 1, Ub, Ua);
+        }
+    }
+    for (; k < nrows; ++k) {
+        if (!step(k, Ua, Ub)) return;
+        Ua = Ub;
     }
 }
 
