@@ -117,14 +117,27 @@ __global__ __launch_bounds__(128) void k_step_fused_redo(View g0, View g1, Fused
         const int longest = max(a.seg, a.seg2);
         const int pieces = max(1, min((longest + kRedoRows - 1) / kRedoRows, (kRedoPieces + n - 1) / n));
         const int rows = (longest + pieces - 1) / pieces;
+        // The exact solver's tiles are on the list for a non-uniform face,
+        // which says nothing of their range: a piece first takes the full
+        // solver with the fast division sequences (kExactFull, bit-identical
+        // where its verdict holds) and the IEEE pass only if it fails.
+        const bool fast = S == kExact && dt_in_fast_range(dt / a.dx) && dt_in_fast_range(dt / a.dy);
         for (int q = blockIdx.x; q < n * pieces; q += gridDim.x) {
             int cg, j0, j1;
             tile_span(a, in.ny, a.redo[2 + q / pieces], cg, j0, j1);
             const int p0 = j0 + (q % pieces) * rows;
             if (p0 < j1) {
-                bool ok = true;
+                bool ok = false;
                 double m = 0.0;
-                fused_rows<S, false>(in, out, a, cg, p0, min(p0 + rows, j1), dt, X, w, lane, ok, m);
+                if constexpr (S == kExact) {
+                    if (fast) {
+                        fused_rows<kExactFull, true>(in, out, a, cg, p0, min(p0 + rows, j1), dt, X, w, lane, ok, m);
+                        if (w == 1 && lane == 0) X.verdict = ok ? 1u : 0u;
+                        pair_barrier();
+                        ok = X.verdict != 0u;
+                    }
+                }
+                if (!ok) fused_rows<S, false>(in, out, a, cg, p0, min(p0 + rows, j1), dt, X, w, lane, ok, m);
                 if (w == 1 && (a.speed || a.clock_speed)) {
                     m = wave_max(m);
                     if (lane == 0) atomic_max_pos(speed_slot(a), m);

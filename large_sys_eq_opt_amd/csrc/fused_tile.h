@@ -56,6 +56,7 @@ struct PairLds {
     unsigned key[64];        // y wavefront: lane's running range key
     unsigned xok;            // x wavefront's fast-range verdict for the tile
     unsigned stop[2];        // exact solver's fast pass: stop after barrier b if stop[b & 1]
+    unsigned verdict;        // redo pass: the y wavefront's verdict on a piece, for both
 };
 constexpr int kPairWindow = 32;
 
@@ -196,7 +197,7 @@ __device__ __forceinline__ void pair_x(const View& in, int c0, int j0, int j1, d
         return true;
     };
     int k = 0;
-    if constexpr (S != kExact) {
+    if constexpr (S != kExact && S != kExactFull) {
         // the window pass falls on even rows only (kPairWindow is even)
         for (; k + 1 < nrows; k += 2) {
             step(k, Ua, Ub);
@@ -293,7 +294,7 @@ __device__ __forceinline__ bool pair_y(const View& out, int c0, int j0, int j1, 
     };
     Cons Fq;
     int p = j0 + 2;
-    if constexpr (S != kExact) {
+    if constexpr (S != kExact && S != kExactFull) {
         for (; p + 1 <= j1 + 1; p += 2) {
             if (!step(p, W0, W1, R, Rp, Fp, Fq)) return false;
             if (!step(p + 1, W1, W0, Rp, R, Fq, Fp)) return false;
