@@ -257,7 +257,10 @@ def test_fused_exact_equals_sweeps(hip, oracle, case, nx, ny, steps, shape):
 def test_fused_exact_mixed_range(hip, oracle):
     """The two-blob field (part of it outside the fast range, where the
     whole tile goes to the redo pass) under the exact solver: fused = two
-    sweeps bit for bit, oracle to 1e-10."""
+    sweeps bit for bit, oracle to 1e-10.  The redo pass runs each piece
+    with the fast division sequences first; the pieces that reach the
+    out-of-range blob fail that verdict and are solved again with the IEEE
+    sequences, the others keep the first result."""
     from large_sys_eq_opt_amd import cases
 
     nx, ny, steps = 192, 150, 9
